@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Benchmark of the fused D3Q15 time step on one MI355X.
+
+Workload (the configuration the D3Q15 MLUPS metric is quoted on): D3Q15
+BGK + Smagorinsky (C_s = 0.17), tau = 0.51, 256^3 periodic box, decaying
+turbulence from a seeded solenoidal random-phase velocity field with
+E(k) ~ k^4 exp(-2 (k/k0)^2), k0 = 4, per-component u_rms = 0.05, f = f_eq
+(rho = 1, u) at t = 0; FP32 populations.  A step is one call of the fused
+kernel over all 256^3 nodes, with the populations resident in HBM.
+
+Timing: W untimed warm-up steps, then exactly K steps between HIP events
+recorded on the library's stream (the stream the kernel runs on), with a
+stream synchronisation before and after.  One process, one GPU.
+
+Printed: one JSON line with MLUPS, the roofline of the step kernel
+(121 algorithmic bytes per node per step: 15 FP32 populations read and
+written, plus the 1-byte node mask), and the FP64 CPU oracle timed on a
+bounded sample of the same workload on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "lbm-c_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "step_d3q15_f32_256_traffic.json")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_lattice(lbmc, L, storage, fields):
+    import numpy as np
+
+    ux, uy, uz = fields
+    lat = lbmc.Lattice(lbmc.D3Q15, (L, L, L), lbmc.BGK, tau=0.51, c_smag=0.17, storage=storage)
+    lat.set_fields(geometry=np.zeros(L ** 3))
+    lat.init_equilibrium(np.ones(L ** 3), (ux, uy, uz))
+    return lat
+
+
+def cpu_baseline(L, fields, seconds, threads):
+    """FP64 oracle (the CPU restatement of the reference step) on the same
+    workload: whole lattice, as many steps as fit in `seconds` (at least 2)."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    pyoracle.set_threads(threads)
+    o = pyoracle.Oracle(pyoracle.D3Q15, (L, L, L), pyoracle.BGK, tau=0.51, c_smag=0.17)
+    o.set_fields(geometry=np.zeros(L ** 3))
+    o.init_equilibrium(np.ones(L ** 3), fields)
+    o.step(0, False)  # warm-up
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        o.step(1 + steps, False)
+        steps += 1
+        el = time.perf_counter() - t0
+        if steps >= 2 and el >= seconds:
+            break
+    o.close()
+    return {"value": round(L ** 3 * steps / el / 1e6, 3), "unit": "MLUPS",
+            "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle, same D3Q15 BGK+Smagorinsky 256^3 lattice and initial state, "
+                      f"{steps} steps in {el:.1f} s on {threads} OpenMP threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-variants", action="store_true", help="skip the FP16S/BF16S lines")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the time step "
+              "has no multi-GPU path in this build", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import lbmc
+    import workloads
+
+    L = args.size
+    N = L ** 3
+    log(f"bench: generating 256^3 turbulence field (seed 2024)" if L == 256 else
+        f"bench: generating {L}^3 turbulence field (seed 2024)")
+    fields = workloads.decaying_turbulence(L, k0=4.0, u_rms=0.05, seed=2024)
+    fields = tuple(v.reshape(-1) for v in fields)
+
+    # bandwidth reference: a plain 1 GiB copy (reads + writes 1 GiB each)
+    nbytes = 1 << 30
+    reps = 20
+    copy_ms = lbmc.copy_bandwidth(nbytes, reps)
+    copy_gbs = 2.0 * nbytes * reps / (copy_ms * 1e-3) / 1e9
+    log(f"bench: copy kernel {copy_gbs:.0f} GB/s")
+
+    log(f"bench: FP32 lattice {L}^3, warmup {args.warmup}, timed {args.steps}")
+    lat = make_lattice(lbmc, L, lbmc.FP32, fields)
+    lat.time_steps(0, args.warmup, 0)
+    lat.sync()
+    wall0 = time.perf_counter()
+    ms = lat.time_steps(args.warmup, args.steps, 0)
+    lat.sync()
+    wall = time.perf_counter() - wall0
+    lat.step(args.warmup + args.steps, True)
+    rho, u = lat.get_macros()
+    finite = bool(np.all(np.isfinite(rho)) and np.all(np.isfinite(u)))
+    lat.close()
+
+    ms_step = ms / args.steps
+    mlups = N * args.steps / (ms * 1e-3) / 1e6
+    bytes_node = 2 * 15 * 4 + 1
+    achieved = bytes_node * N / (ms_step * 1e-3) / 1e9
+    log(f"bench: {ms_step:.4f} ms/step, {mlups:.0f} MLUPS, {achieved:.0f} GB/s algorithmic, "
+        f"wall {wall:.3f} s, finite={finite}")
+
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+
+    variants = {}
+    if not args.no_variants:
+        for st in (lbmc.FP16S, lbmc.BF16S):
+            v = make_lattice(lbmc, L, st, fields)
+            v.time_steps(0, max(5, args.warmup // 5), 0)
+            vms = v.time_steps(args.warmup, max(20, args.steps // 5), 0)
+            vsteps = max(20, args.steps // 5)
+            vb = 2 * 15 * 2 + 1
+            variants[lbmc.STORAGE_NAMES[st]] = {
+                "MLUPS": round(N * vsteps / (vms * 1e-3) / 1e6, 1),
+                "ms_per_step": round(vms / vsteps, 5),
+                "bytes_per_node": vb,
+                "achieved_GBs": round(vb * N / (vms / vsteps * 1e-3) / 1e9, 1)}
+            v.close()
+            log(f"bench: {lbmc.STORAGE_NAMES[st]} {variants[lbmc.STORAGE_NAMES[st]]}")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"bench: CPU oracle baseline, {threads} threads, ~{args.cpu_seconds} s")
+        cpu = cpu_baseline(L, fields, args.cpu_seconds, threads)
+        log(f"bench: CPU {cpu}")
+
+    out = {
+        "metric": "D3Q15 MLUPS",
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"D3Q15 BGK+Smagorinsky decaying turbulence {L}^3",
+                   "lattice": "D3Q15", "dims": [L, L, L], "tau": 0.51, "c_smag": 0.17,
+                   "storage": "f32", "init": "f_eq(1, u), solenoidal k^4 exp(-2(k/4)^2), "
+                   "u_rms 0.05, seed 2024", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "bytes_per_node": bytes_node,
+                     "copy_kernel_GBs": round(copy_gbs, 1)},
+        "cpu_baseline": cpu,
+        "variants": variants,
+        "finite": finite,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

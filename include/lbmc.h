@@ -1,0 +1,172 @@
+/*
+ * lbmc.h -- C ABI of the MI355X lattice Boltzmann time-step library.
+ *
+ * The library replaces LBM-C's device time-step path and the host glue that
+ * drives it:
+ *   - the kernel iterate_kernel (LBM-C/solver.cu:10-136) and its launcher
+ *     iterate(int t) (LBM-C/index.cu:284-313)        -> lbmc_step / lbmc_run
+ *   - the constant upload cudaMemcpyToSymbol("domain_constants")
+ *     (LBM-C/model_builder.cu:63-83)                  -> lbmc_create
+ *   - the device arrays and pointer-of-pointer structs built by
+ *     ModelBuilder (LBM-C/model_builder.cu:86-231)    -> lbmc_set_fields
+ *   - load_static_IC (LBM-C/model_builder.cu:233-248) -> lbmc_init_static
+ *   - the read-backs of output_macros / screen_mess
+ *     (LBM-C/index.cu:196-216, :412-439)              -> lbmc_get_macros / lbmc_get_node
+ *   - current_RMS's Thrust energy reduction
+ *     (LBM-C/index.cu:315-370)                        -> lbmc_kinetic_energy
+ *   - combi_malloc / cudasafe / Check_CUDA_Error
+ *     (LBM-C/cuda_util.cu:6-27)                       -> status codes + lbmc_last_error
+ *
+ * Conventions
+ *   - Plain C types only.  Host arrays use the reference's layout: one array
+ *     per component, N = Lx*Ly(*Lz) nodes, x fastest (solver.cu:23,26).
+ *   - Every function returns an lbmc_status; LBMC_OK is 0.  The reference
+ *     printed and called exit(-1) on any error; a caller that wants that
+ *     behaviour checks the status and does so itself.
+ *   - The context owns all device memory; the caller owns host buffers.
+ *   - One host thread per context.  Work is ordered on the context's stream;
+ *     lbmc_get_* and lbmc_sync wait for it.
+ */
+#ifndef LBMC_H
+#define LBMC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum lbmc_status {
+    LBMC_OK = 0,
+    LBMC_ERR_INVALID = 1,     /* bad argument or field value */
+    LBMC_ERR_NOMEM = 2,       /* device or host allocation failed */
+    LBMC_ERR_DEVICE = 3,      /* HIP runtime or kernel launch error */
+    LBMC_ERR_STATE = 4,       /* call out of order (e.g. step before fields) */
+    LBMC_ERR_UNSUPPORTED = 5  /* valid request this build does not offer */
+} lbmc_status;
+
+/* lattice: the compile-time D2Q9 / D3Q15 switch of LBM-C/macros.cu:11-12 */
+enum { LBMC_D2Q9 = 0, LBMC_D3Q15 = 1 };
+
+/* ColType codes of the .lbmc file (LBM-C/infile_reader.cu:118-144) */
+enum { LBMC_BGK = 0, LBMC_NTPOR = 1, LBMC_MRT = 2, LBMC_MRTPOR = 3 };
+
+/* population storage: the reference is FP64 throughout
+ * (LBM-C/data_types.cuh:7-11); the others are this library's additions.
+ * FP16S/BF16S store f_i - w_i and compute in FP32. */
+enum { LBMC_FP64 = 0, LBMC_FP32 = 1, LBMC_FP16S = 2, LBMC_BF16S = 3 };
+
+/* Fixes for defects of the reference.  0 (the default) reproduces the
+ * reference exactly. */
+#define LBMC_FIX_D3Q15_MINV 0x1u /* D3Q15 M^-1 is all zeros (LBM-C/macros.cu:99-113) */
+#define LBMC_FIX_D3Q15_MEQ 0x2u  /* (-7/3) is integer -2 (LBM-C/collision.cu:492-496) */
+#define LBMC_FIX_D3Q15_ZH 0x4u   /* D3Q15 Zou-He leaves transverse momentum and, on
+                                    face Z, mass (LBM-C/boundary_conditions/d3q15/
+                                    d3q15_zh_defs.cu:4-92) */
+
+/* The run-time constants of DomainConstant (LBM-C/data_types.cuh:23-43) that
+ * the step uses, plus this library's storage choice. */
+typedef struct lbmc_desc {
+    int lattice;         /* LBMC_D2Q9 or LBMC_D3Q15 */
+    int length[3];       /* Lx, Ly, Lz (Lz ignored for D2Q9) */
+    int collision_type;  /* LBMC_BGK .. LBMC_MRTPOR */
+    int forcing;         /* Guo body force on (the Force key) */
+    int macro_bc;        /* per-node macro BC codes on (MacroBC key) */
+    int micro_bc;        /* per-node micro BC codes on (MicroBC key) */
+    double tau;          /* BGK relaxation time (Tau key) */
+    double tau_mrt[15];  /* MRT relaxation rates S_i (TauMRT key) */
+    double c_smag;       /* Smagorinsky constant; > 0 enables it (C_smag key) */
+    int storage;         /* LBMC_FP64 .. LBMC_BF16S */
+    unsigned int quirks; /* LBMC_FIX_* flags */
+    int device;          /* HIP device ordinal (the reference hard-codes 0) */
+} lbmc_desc;
+
+typedef struct lbmc_ctx lbmc_ctx;
+
+/* Allocate the populations (Q x N slots), macro arrays and node mask on the
+ * device.  Replaces constant_loader + variable_size_allocator
+ * (LBM-C/model_builder.cu:63-145). */
+lbmc_status lbmc_create(const lbmc_desc *desc, lbmc_ctx **out);
+
+/* Release everything the context owns (the reference never frees). */
+lbmc_status lbmc_destroy(lbmc_ctx *ctx);
+
+/* Message of the last failed call on ctx; ctx may be NULL for lbmc_create. */
+const char *lbmc_last_error(const lbmc_ctx *ctx);
+
+/* Per-node fields in the reference's host layout (variable_loader,
+ * LBM-C/model_builder.cu:173-225): geometry = the "Porosity" field
+ * (B >= 1 is solid), macro_bc / micro_bc = integer codes (used when the
+ * descriptor enables them), force[d] = ForceX/Y/Z (when forcing), u[d] and
+ * rho = VelocityX/Y/Z and Rho: the prescribed values at macro-BC nodes, which
+ * are also the initial contents of the macro output arrays.  Any pointer may
+ * be NULL (zeros).  Builds the 1-byte node mask and the class table.
+ * Invalid codes (outside 0..15 for macro; outside the lattice's micro table
+ * or a NULL slot of it, LBM-C/boundary_conditions/micro_bc.cuh:7-8,14-15)
+ * return LBMC_ERR_INVALID. */
+lbmc_status lbmc_set_fields(lbmc_ctx *ctx, const double *geometry, const int *macro_bc,
+                            const int *micro_bc, const double *const *force,
+                            const double *const *u, const double *rho);
+
+/* The node mask (N bytes: class id per node) and the class table
+ * (256 uint16 descriptors; *nclasses = number in use).  Descriptor bits:
+ * 0 solid, 1 porous, 4-7 macro code, 8-11 micro code.  Class 0 is plain
+ * fluid; the others are the distinct nonzero descriptors in increasing
+ * order. */
+lbmc_status lbmc_get_node_types(lbmc_ctx *ctx, uint8_t *mask, uint16_t *table, int *nclasses);
+
+/* Bounceback link sets: links[n] bit i is set when node n is not solid and
+ * its periodic neighbour n + e_i is solid.  Computed on the device. */
+lbmc_status lbmc_get_links(lbmc_ctx *ctx, uint16_t *links);
+
+/* f_i = w_i at every node (load_static_IC, LBM-C/model_builder.cu:233-248). */
+lbmc_status lbmc_init_static(lbmc_ctx *ctx);
+
+/* f_i = feq_i(rho, u) at every node (the BGK equilibrium of
+ * LBM-C/collision.cu:53-59); an addition: the reference has no such IC. */
+lbmc_status lbmc_init_equilibrium(lbmc_ctx *ctx, const double *rho, const double *const *u);
+
+/* One call of iterate_kernel with time index t (LBM-C/index.cu:108 passes
+ * i-1 for step i), storing rho/u when store_macros is nonzero. */
+lbmc_status lbmc_step(lbmc_ctx *ctx, long long t, int store_macros);
+
+/* nsteps calls with t = t0 .. t0+nsteps-1; macros are stored on the calls
+ * where (t+1) % store_every == 0 (store_every <= 0: never). */
+lbmc_status lbmc_run(lbmc_ctx *ctx, long long t0, int nsteps, int store_every);
+
+/* As lbmc_run, timed with HIP events on the context's stream; *ms is the
+ * device time of the nsteps calls. */
+lbmc_status lbmc_time_steps(lbmc_ctx *ctx, long long t0, int nsteps, int store_every, float *ms);
+
+/* The stored macro arrays (as output_macros copies them back). */
+lbmc_status lbmc_get_macros(lbmc_ctx *ctx, double *rho, double *const *u);
+
+/* One node's stored rho and u (screen_mess, LBM-C/index.cu:412-439). */
+lbmc_status lbmc_get_node(lbmc_ctx *ctx, long long idx, double *rho, double *u);
+
+/* Raw population slots f[i][slot] as doubles (w_i + g_i for shifted
+ * storage), for slot-by-slot comparison and checkpoints. */
+lbmc_status lbmc_get_f(lbmc_ctx *ctx, double *const *f);
+lbmc_status lbmc_set_f(lbmc_ctx *ctx, const double *const *f);
+
+/* E = sum_n 1/2 rho |u|^2 over the stored macro arrays (the energy functor
+ * and thrust::reduce of LBM-C/index.cu:315-370), accumulated in FP64 in a
+ * fixed order. */
+lbmc_status lbmc_kinetic_energy(lbmc_ctx *ctx, double *energy);
+
+/* Wait for all work on the context's stream. */
+lbmc_status lbmc_sync(lbmc_ctx *ctx);
+
+/* Information: node count, Q, bytes of one population slot. */
+lbmc_status lbmc_info(const lbmc_ctx *ctx, long long *nodes, int *q, int *slot_bytes);
+
+/* Plain device-to-device copy of nbytes (multiple of 16) inside a scratch
+ * allocation, timed like lbmc_time_steps; the bandwidth reference for the
+ * roofline and the HBM-counter calibration.  Not part of the reference. */
+lbmc_status lbmc_copy_bandwidth(int device, size_t nbytes, int reps, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

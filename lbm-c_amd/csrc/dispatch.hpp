@@ -1,0 +1,49 @@
+// Host-side launchers of the device kernels (one translation unit per
+// population storage format, so that the FP64 unit can be built with
+// contraction off).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "step_args.hpp"
+
+namespace lbmc {
+
+// fused step, step_<storage>.hip
+hipError_t launch_step_fp64(const KernelKey &k, const StepArgs<double> &a, int grid, hipStream_t s);
+hipError_t launch_step_fp32(const KernelKey &k, const StepArgs<float> &a, int grid, hipStream_t s);
+hipError_t launch_step_fp16s(const KernelKey &k, const StepArgs<float> &a, int grid, hipStream_t s);
+hipError_t launch_step_bf16s(const KernelKey &k, const StepArgs<float> &a, int grid, hipStream_t s);
+
+// sf neighbour gather, step_<storage>.hip
+hipError_t launch_sf_gather_fp64(int lattice, const StepArgs<double> &a, const int *sfnb, int nsf,
+                                 double *out, hipStream_t s);
+hipError_t launch_sf_gather_fp32(int lattice, const StepArgs<float> &a, const int *sfnb, int nsf,
+                                 float *out, hipStream_t s);
+hipError_t launch_sf_gather_fp16s(int lattice, const StepArgs<float> &a, const int *sfnb, int nsf,
+                                  float *out, hipStream_t s);
+hipError_t launch_sf_gather_bf16s(int lattice, const StepArgs<float> &a, const int *sfnb, int nsf,
+                                  float *out, hipStream_t s);
+
+// auxiliary kernels, aux.hip
+hipError_t launch_init_static(int lattice, int storage, void *f, long long npad, long long n,
+                              hipStream_t s);
+hipError_t launch_init_equilibrium(int lattice, int storage, void *f, long long npad, long long n,
+                                   const double *rho, const double *ux, const double *uy,
+                                   const double *uz, hipStream_t s);
+// direction i of f <-> double[n]
+hipError_t launch_f_to_double(int lattice, int storage, const void *f, long long npad, int i,
+                              long long n, double *out, hipStream_t s);
+hipError_t launch_f_from_double(int lattice, int storage, void *f, long long npad, int i,
+                                long long n, const double *in, hipStream_t s);
+// E partials: one double per block; then the ordered final sum
+hipError_t launch_energy(int dim, int cdouble, const void *rho, const void *ux, const void *uy,
+                         const void *uz, long long n, double *partials, int nblocks, double *out,
+                         hipStream_t s);
+hipError_t launch_links(int lattice, const uint8_t *mask, const uint16_t *cls, int Lx, int Ly, int Lz,
+                        uint16_t *links, hipStream_t s);
+hipError_t launch_copy(const void *src, void *dst, size_t nbytes, hipStream_t s);
+
+}  // namespace lbmc

@@ -1,0 +1,55 @@
+// Kernel argument block shared by the host dispatcher and the step kernels.
+#pragma once
+
+#include <cstdint>
+
+namespace lbmc {
+
+// threads per block of the step kernels
+constexpr int kBlock = 256;
+
+// which fused kernel variant a launch needs
+struct KernelKey {
+    int lattice;  // 0 D2Q9, 1 D3Q15
+    int family;   // 0 BGK (BGK and NTPOR), 1 MRT (MRT and MRTPOR)
+    int guo;      // Guo body force
+    int smag;     // Smagorinsky (BGK family only)
+    int mfix;     // corrected D3Q15 M^-1 (MRT family only)
+};
+
+enum : unsigned {
+    QK_FIX_D3Q15_MINV = 0x1u,
+    QK_FIX_D3Q15_MEQ = 0x2u,
+    QK_FIX_D3Q15_ZH = 0x4u,
+};
+
+template <class C>
+struct StepArgs {
+    void *f;                 // Q directions x npad slots of the storage type
+    long long npad;          // elements between direction arrays
+    const uint8_t *mask;     // node class per node
+    const uint16_t *cls;     // class descriptors (256 entries)
+    const C *porosity;       // per node; read at porous classes only
+    const C *force[3];       // per-node force, or null for a uniform force
+    C F[3];                  // uniform force
+    C *rho;                  // macro arrays: output, and prescribed values
+    C *u[3];                 //   at macro-BC nodes (LBM-C/boundary_conditions/macro_bc.cu)
+    const C *sfbuf;          // gathered neighbour populations for sf nodes
+    const int *sfidx;        // per node: row of sfbuf (sf nodes only)
+    int L[3];
+    int s[3];                // t mod L[d]: the index-shift of this call
+    int xcap;                // threads along x per row pass
+    int rows_per_pass;       // rows processed concurrently by one block
+    int rows_per_block;
+    int nrows;               // Ly * Lz
+    int xseg;                // x extent of one block (multiple of xcap)
+    int nseg;                // ceil(Lx / xseg): blocks per row group
+    C tau;
+    C c_smag;
+    C tau_mrt[15];
+    int store;               // store rho/u this call
+    int porous;              // NTPOR/MRTPOR with porous nodes present
+    unsigned quirks;
+};
+
+}  // namespace lbmc

@@ -1,0 +1,284 @@
+// The fused pull-stream / boundary / collide / store kernel.
+//
+// One launch performs one call of the reference's iterate_kernel
+// (LBM-C/solver.cu:10-136) for every node:
+//   1. index-shift gather: population i of node x is read from slot
+//      p_i(x,t) = (x - t e_i) mod L in every axis (solver.cu:77-94);
+//   2. rho = sum f_i, u = sum e_i f_i / rho (solver.cu:95-108);
+//   3. node class from the 1-byte mask: solid -> bounceback; otherwise macro
+//      BC, micro BC (solver.cu:111-112) and the collision (solver.cu:115);
+//   4. the post-collision populations go back to the same slots
+//      (solver.cu:119-123), so the update is in place with one buffer and
+//      no ghost layers: each slot is owned by exactly one node per call;
+//   5. rho and u are stored when asked (solver.cu:125-134).
+//
+// Work decomposition: a block owns whole x-rows (or x-segments of long rows);
+// for a fixed direction the slots a row touches are one x-row rotated by
+// t*e_x, so a block that covers a full row reads and writes exactly the cache
+// lines of that row.  No __syncthreads and no LDS are needed: nodes are
+// independent within a call.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "lattice.hpp"
+#include "node_ops.hpp"
+#include "step_args.hpp"
+#include "storage.hpp"
+
+namespace lbmc {
+
+template <class L, class St, int FAM, bool GUO, bool SMAG, bool MFIX>
+__global__ __launch_bounds__(kBlock) void step_kernel(const StepArgs<typename St::C> a)
+{
+    using C = typename St::C;
+    using S = typename St::S;
+    constexpr int Q = L::Q;
+    constexpr int D = L::D;
+
+    const int Lx = a.L[0], Ly = a.L[1], Lz = a.L[2];
+    const int sx = a.s[0], sy = a.s[1], sz = a.s[2];
+    const int tx = (int)threadIdx.x % a.xcap;
+    const int ty = (int)threadIdx.x / a.xcap;
+    if (ty >= a.rows_per_pass) return;
+    const int seg = (int)blockIdx.x % a.nseg;
+    const int row0 = ((int)blockIdx.x / a.nseg) * a.rows_per_block;
+    const int row1 = min(row0 + a.rows_per_block, a.nrows);
+    const int xbeg = seg * a.xseg;
+    const int xend = min(Lx, xbeg + a.xseg);
+    S *const fb = (S *)a.f;
+
+    const C qc = (a.quirks & QK_FIX_D3Q15_MEQ) ? C(-7.0 / 3.0) : C(-2);
+    const bool zhfix = (a.quirks & QK_FIX_D3Q15_ZH) != 0;
+
+    for (int row = row0 + ty; row < row1; row += a.rows_per_pass) {
+        const int y = row % Ly;
+        const int z = row / Ly;
+        // source rows for e_y, e_z in {-1, 0, 1} (index e+1)
+        int yy[3], zz[3];
+        yy[1] = y;
+        yy[2] = y - sy;
+        if (yy[2] < 0) yy[2] += Ly;
+        yy[0] = y + sy;
+        if (yy[0] >= Ly) yy[0] -= Ly;
+        zz[1] = z;
+        zz[2] = z - sz;
+        if (zz[2] < 0) zz[2] += Lz;
+        zz[0] = z + sz;
+        if (zz[0] >= Lz) zz[0] -= Lz;
+
+        for (int x = xbeg + tx; x < xend; x += a.xcap) {
+            int xx[3];
+            xx[1] = x;
+            xx[2] = x - sx;
+            if (xx[2] < 0) xx[2] += Lx;
+            xx[0] = x + sx;
+            if (xx[0] >= Lx) xx[0] -= Lx;
+            const int n = row * Lx + x;
+
+            // ---- gather and moments
+            int idx[Q];
+            C f[Q];
+            C rho = C(0);
+            C j[3] = {C(0), C(0), C(0)};
+            bool jfirst[3] = {true, true, true};
+            sfor<0, Q>([&](auto ii) {
+                constexpr int i = decltype(ii)::value;
+                idx[i] = (zz[L::ez[i] + 1] * Ly + yy[L::ey[i] + 1]) * Lx + xx[L::ex[i] + 1];
+                const C v = St::load(fb[(size_t)i * a.npad + idx[i]]);
+                rho = (i == 0) ? v : rho + v;
+                sfor<0, D>([&](auto dd) {
+                    constexpr int d = decltype(dd)::value;
+                    constexpr int e = e_of<L>(d, i);
+                    if constexpr (e != 0) {
+                        if (jfirst[d]) j[d] = (e > 0) ? v : -v;
+                        else j[d] = (e > 0) ? j[d] + v : j[d] - v;
+                        jfirst[d] = false;
+                    }
+                });
+                if constexpr (St::shifted) f[i] = v + C(L::w[i]);
+                else f[i] = v;
+            });
+            if constexpr (St::shifted) rho = C(1) + rho;
+            C u[3];
+            sfor<0, D>([&](auto dd) {
+                constexpr int d = decltype(dd)::value;
+                u[d] = j[d] / rho;
+            });
+
+            // ---- node class
+            const unsigned m = a.mask[n];
+            const uint16_t desc = m ? a.cls[m] : (uint16_t)0;
+
+            if (desc & CLS_SOLID) {
+                // bounceback, collision.cu:423-443
+                C t[Q];
+                sfor<0, Q>([&](auto ii) {
+                    constexpr int i = decltype(ii)::value;
+                    t[i] = f[L::opp[i]];
+                });
+                sfor<0, Q>([&](auto ii) {
+                    constexpr int i = decltype(ii)::value;
+                    f[i] = t[i];
+                });
+                rho = C(0);
+                sfor<0, D>([&](auto dd) { u[decltype(dd)::value] = C(0); });
+            } else {
+                C B = C(0);
+                if (desc) {
+                    if (desc & CLS_POROUS) B = a.porosity[n];
+                    const int mc = cls_macro(desc);
+                    if (mc) {
+                        // presc_*, macro_bc.cu:19-125
+                        const unsigned ps = macro_set(mc);
+                        if (ps & 1u) rho = a.rho[n];
+                        sfor<0, D>([&](auto dd) {
+                            constexpr int d = decltype(dd)::value;
+                            if (ps & (2u << d)) u[d] = a.u[d][n];
+                        });
+                    }
+                    const int mi = cls_micro(desc);
+                    if (mi) {
+                        const C *nb = a.sfidx ? a.sfbuf + (size_t)a.sfidx[n] * Q : nullptr;
+                        apply_micro<L>(mi, f, rho, u, nb, zhfix);
+                    }
+                }
+
+                C F[3] = {C(0), C(0), C(0)};
+                if constexpr (GUO) {
+                    if (a.force[0]) {
+                        sfor<0, D>([&](auto dd) {
+                            constexpr int d = decltype(dd)::value;
+                            F[d] = a.force[d][n];
+                        });
+                    } else {
+                        sfor<0, D>([&](auto dd) {
+                            constexpr int d = decltype(dd)::value;
+                            F[d] = a.F[d];
+                        });
+                    }
+                }
+
+                if constexpr (FAM == 0) {
+                    // bgk / bgk_guo / bgk_ntpor / bgk_ntpor_guo, collision.cu:49-190
+                    C ft[Q];
+                    C tau = a.tau;
+                    if constexpr (GUO) {
+                        guo_shift<L>(rho, F, u);
+                        guo_source<L>(tau, u, F, ft);
+                    }
+                    C feq[Q];
+                    equilibrium<L>(rho, u, feq);
+                    if constexpr (SMAG) {
+                        if (a.c_smag > C(0)) tau = smagorinsky_tau<L>(tau, rho, a.c_smag, f, feq);
+                    }
+                    const C omega = C(1.0) / tau;
+                    if (a.porous) {
+                        const C ob = C(1) - B;
+                        C t[Q];
+                        sfor<0, Q>([&](auto ii) {
+                            constexpr int i = decltype(ii)::value;
+                            const C cb = omega * (f[i] - feq[i]);
+                            const C cs = f[L::opp[i]] - f[i];
+                            C r = f[i] - ob * cb + B * cs;
+                            if constexpr (GUO) r = r + ob * ft[i];
+                            t[i] = r;
+                        });
+                        sfor<0, Q>([&](auto ii) {
+                            constexpr int i = decltype(ii)::value;
+                            f[i] = t[i];
+                        });
+                    } else {
+                        sfor<0, Q>([&](auto ii) {
+                            constexpr int i = decltype(ii)::value;
+                            C r = f[i] - omega * (f[i] - feq[i]);
+                            if constexpr (GUO) r = r + ft[i];
+                            f[i] = r;
+                        });
+                    }
+                } else {
+                    // mrt / mrt_guo / mrt_ntpor / mrt_ntpor_guo, collision.cu:192-421
+                    C ft[Q];
+                    if constexpr (GUO) {
+                        guo_shift<L>(rho, F, u);
+                        guo_source<L>(a.tau, u, F, ft);
+                    }
+                    C meq[Q], mm[Q];
+                    moment_equilibrium<L>(rho, u, qc, meq);
+                    mrt_moments<L>(f, meq, a.tau_mrt, mm);
+                    if (a.porous) {
+                        // in place: f[opp[i]] is already post-collision when
+                        // opp[i] < i (collision.cu:331-343, :406-419)
+                        const C ob = C(1) - B;
+                        sfor<0, Q>([&](auto ii) {
+                            constexpr int i = decltype(ii)::value;
+                            const C acc = minv_row<L, MFIX, i>(mm);
+                            const C cs = f[L::opp[i]] - f[i];
+                            C r = f[i] - ob * acc;
+                            if constexpr (GUO) r = r + ob * ft[i];
+                            f[i] = r + B * cs;
+                        });
+                    } else {
+                        sfor<0, Q>([&](auto ii) {
+                            constexpr int i = decltype(ii)::value;
+                            C r = f[i] - minv_row<L, MFIX, i>(mm);
+                            if constexpr (GUO) r = r + ft[i];
+                            f[i] = r;
+                        });
+                    }
+                }
+            }
+
+            // ---- scatter to the same slots
+            sfor<0, Q>([&](auto ii) {
+                constexpr int i = decltype(ii)::value;
+                C v = f[i];
+                if constexpr (St::shifted) v = v - C(L::w[i]);
+                fb[(size_t)i * a.npad + idx[i]] = St::store(v);
+            });
+
+            if (a.store) {
+                sfor<0, D>([&](auto dd) {
+                    constexpr int d = decltype(dd)::value;
+                    a.u[d][n] = u[d];
+                });
+                a.rho[n] = rho;
+            }
+        }
+    }
+}
+
+// Gathers the post-stream populations of the neighbours of the sf nodes
+// before the step kernel overwrites any slot.  sfnodes[k] is a node index,
+// sfnb[k] its neighbour's index.
+template <class L, class St>
+__global__ __launch_bounds__(kBlock) void sf_gather_kernel(const StepArgs<typename St::C> a,
+                                                           const int *sfnb, int nsf, typename St::C *out)
+{
+    using C = typename St::C;
+    using S = typename St::S;
+    constexpr int Q = L::Q;
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= nsf) return;
+    const int Lx = a.L[0], Ly = a.L[1], Lz = a.L[2];
+    const int nb = sfnb[k];
+    const int x = nb % Lx;
+    const int y = (nb / Lx) % Ly;
+    const int z = nb / (Lx * Ly);
+    const S *fb = (const S *)a.f;
+    sfor<0, Q>([&](auto ii) {
+        constexpr int i = decltype(ii)::value;
+        int xs = x - L::ex[i] * a.s[0];
+        xs = xs < 0 ? xs + Lx : (xs >= Lx ? xs - Lx : xs);
+        int ys = y - L::ey[i] * a.s[1];
+        ys = ys < 0 ? ys + Ly : (ys >= Ly ? ys - Ly : ys);
+        int zs = z - L::ez[i] * a.s[2];
+        zs = zs < 0 ? zs + Lz : (zs >= Lz ? zs - Lz : zs);
+        const int id = (zs * Ly + ys) * Lx + xs;
+        C v = St::load(fb[(size_t)i * a.npad + id]);
+        if constexpr (St::shifted) v = v + C(L::w[i]);
+        out[(size_t)k * Q + i] = v;
+    });
+}
+
+}  // namespace lbmc

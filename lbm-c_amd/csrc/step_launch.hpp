@@ -1,0 +1,60 @@
+// Variant selection for the fused step: one instantiation per lattice,
+// collision family, Guo forcing, Smagorinsky and D3Q15 M^-1 choice.
+#pragma once
+
+#include "dispatch.hpp"
+#include "step_kernel.hpp"
+
+namespace lbmc {
+
+template <class L, class St, int FAM, bool GUO, bool SMAG, bool MFIX>
+static hipError_t launch_one(const StepArgs<typename St::C> &a, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL((step_kernel<L, St, FAM, GUO, SMAG, MFIX>), dim3(grid), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+template <class L, class St>
+static hipError_t launch_lattice(const KernelKey &k, const StepArgs<typename St::C> &a, int grid,
+                                 hipStream_t s)
+{
+    if (k.family == 0) {
+        if (k.guo) {
+            return k.smag ? launch_one<L, St, 0, true, true, false>(a, grid, s)
+                          : launch_one<L, St, 0, true, false, false>(a, grid, s);
+        }
+        return k.smag ? launch_one<L, St, 0, false, true, false>(a, grid, s)
+                      : launch_one<L, St, 0, false, false, false>(a, grid, s);
+    }
+    if constexpr (L::D == 3) {
+        if (k.mfix) {
+            return k.guo ? launch_one<L, St, 1, true, false, true>(a, grid, s)
+                         : launch_one<L, St, 1, false, false, true>(a, grid, s);
+        }
+    }
+    return k.guo ? launch_one<L, St, 1, true, false, false>(a, grid, s)
+                 : launch_one<L, St, 1, false, false, false>(a, grid, s);
+}
+
+template <class St>
+static hipError_t launch_step_st(const KernelKey &k, const StepArgs<typename St::C> &a, int grid,
+                                 hipStream_t s)
+{
+    return k.lattice == 0 ? launch_lattice<D2Q9, St>(k, a, grid, s)
+                          : launch_lattice<D3Q15, St>(k, a, grid, s);
+}
+
+template <class St>
+static hipError_t launch_sf_st(int lattice, const StepArgs<typename St::C> &a, const int *sfnb,
+                               int nsf, typename St::C *out, hipStream_t s)
+{
+    if (nsf <= 0) return hipSuccess;
+    const int grid = (nsf + kBlock - 1) / kBlock;
+    if (lattice == 0)
+        hipLaunchKernelGGL((sf_gather_kernel<D2Q9, St>), dim3(grid), dim3(kBlock), 0, s, a, sfnb, nsf, out);
+    else
+        hipLaunchKernelGGL((sf_gather_kernel<D3Q15, St>), dim3(grid), dim3(kBlock), 0, s, a, sfnb, nsf, out);
+    return hipGetLastError();
+}
+
+}  // namespace lbmc

@@ -1,0 +1,294 @@
+"""ctypes binding of the C ABI in include/lbmc.h (liblbmc.so, built in-tree).
+
+The binding mirrors the reference's host-side vocabulary: a lattice is
+created from the run-time constants the reference keeps in DomainConstant
+(LBM-C/data_types.cuh:23-43), fed the per-node fields ModelBuilder loads
+(LBM-C/model_builder.cu:173-225) and stepped with the reference's time index
+(LBM-C/index.cu:102-108).  Arrays are numpy arrays with N = Lx*Ly(*Lz)
+entries in the reference's order (x fastest); a (Lz, Ly, Lx) or (Ly, Lx)
+C-ordered array is that order.
+
+There is no fallback: if liblbmc.so is missing, importing this module raises.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblbmc.so")
+
+# enums of include/lbmc.h
+D2Q9, D3Q15 = 0, 1
+BGK, NTPOR, MRT, MRTPOR = 0, 1, 2, 3
+FP64, FP32, FP16S, BF16S = 0, 1, 2, 3
+FIX_D3Q15_MINV, FIX_D3Q15_MEQ, FIX_D3Q15_ZH = 0x1, 0x2, 0x4
+STORAGE_NAMES = {FP64: "f64", FP32: "f32", FP16S: "f16s", BF16S: "bf16s"}
+SLOT_BYTES = {FP64: 8, FP32: 4, FP16S: 2, BF16S: 2}
+
+STATUS = {0: "LBMC_OK", 1: "LBMC_ERR_INVALID", 2: "LBMC_ERR_NOMEM", 3: "LBMC_ERR_DEVICE",
+          4: "LBMC_ERR_STATE", 5: "LBMC_ERR_UNSUPPORTED"}
+
+EXPORTS = [
+    "lbmc_create", "lbmc_destroy", "lbmc_last_error", "lbmc_set_fields", "lbmc_get_node_types",
+    "lbmc_get_links", "lbmc_init_static", "lbmc_init_equilibrium", "lbmc_step", "lbmc_run",
+    "lbmc_time_steps", "lbmc_get_macros", "lbmc_get_node", "lbmc_get_f", "lbmc_set_f",
+    "lbmc_kinetic_energy", "lbmc_sync", "lbmc_info", "lbmc_copy_bandwidth",
+]
+
+
+class LbmcError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+class Desc(ct.Structure):
+    _fields_ = [
+        ("lattice", ct.c_int),
+        ("length", ct.c_int * 3),
+        ("collision_type", ct.c_int),
+        ("forcing", ct.c_int),
+        ("macro_bc", ct.c_int),
+        ("micro_bc", ct.c_int),
+        ("tau", ct.c_double),
+        ("tau_mrt", ct.c_double * 15),
+        ("c_smag", ct.c_double),
+        ("storage", ct.c_int),
+        ("quirks", ct.c_uint),
+        ("device", ct.c_int),
+    ]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                          "(make -C lbm-c_amd)")
+    lib = ct.CDLL(LIB_PATH)
+    P = ct.c_void_p
+    dpp = ct.POINTER(ct.POINTER(ct.c_double))
+    sig = {
+        "lbmc_create": (ct.c_int, [ct.POINTER(Desc), ct.POINTER(P)]),
+        "lbmc_destroy": (ct.c_int, [P]),
+        "lbmc_last_error": (ct.c_char_p, [P]),
+        "lbmc_set_fields": (ct.c_int, [P, ct.POINTER(ct.c_double), ct.POINTER(ct.c_int),
+                                       ct.POINTER(ct.c_int), dpp, dpp, ct.POINTER(ct.c_double)]),
+        "lbmc_get_node_types": (ct.c_int, [P, ct.POINTER(ct.c_uint8), ct.POINTER(ct.c_uint16),
+                                           ct.POINTER(ct.c_int)]),
+        "lbmc_get_links": (ct.c_int, [P, ct.POINTER(ct.c_uint16)]),
+        "lbmc_init_static": (ct.c_int, [P]),
+        "lbmc_init_equilibrium": (ct.c_int, [P, ct.POINTER(ct.c_double), dpp]),
+        "lbmc_step": (ct.c_int, [P, ct.c_longlong, ct.c_int]),
+        "lbmc_run": (ct.c_int, [P, ct.c_longlong, ct.c_int, ct.c_int]),
+        "lbmc_time_steps": (ct.c_int, [P, ct.c_longlong, ct.c_int, ct.c_int,
+                                       ct.POINTER(ct.c_float)]),
+        "lbmc_get_macros": (ct.c_int, [P, ct.POINTER(ct.c_double), dpp]),
+        "lbmc_get_node": (ct.c_int, [P, ct.c_longlong, ct.POINTER(ct.c_double),
+                                     ct.POINTER(ct.c_double)]),
+        "lbmc_get_f": (ct.c_int, [P, dpp]),
+        "lbmc_set_f": (ct.c_int, [P, dpp]),
+        "lbmc_kinetic_energy": (ct.c_int, [P, ct.POINTER(ct.c_double)]),
+        "lbmc_sync": (ct.c_int, [P]),
+        "lbmc_info": (ct.c_int, [P, ct.POINTER(ct.c_longlong), ct.POINTER(ct.c_int),
+                                 ct.POINTER(ct.c_int)]),
+        "lbmc_copy_bandwidth": (ct.c_int, [ct.c_int, ct.c_size_t, ct.c_int,
+                                           ct.POINTER(ct.c_float)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_lib = _load()
+
+
+def lib():
+    return _lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ct.POINTER(ct.c_double))
+
+
+def _ptr_array(arrays, n):
+    """double*[n] from a list of float64 arrays (None -> NULL)."""
+    arr = (ct.POINTER(ct.c_double) * n)()
+    for k in range(n):
+        a = arrays[k] if arrays is not None and k < len(arrays) else None
+        arr[k] = _dptr(a) if a is not None else ct.POINTER(ct.c_double)()
+    return ct.cast(arr, ct.POINTER(ct.POINTER(ct.c_double)))
+
+
+def _f64(a, n):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float64)).reshape(-1)
+    if a.size != n:
+        raise ValueError(f"array has {a.size} entries, the lattice has {n} nodes")
+    return a
+
+
+def _i32(a, n):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.int32)).reshape(-1)
+    if a.size != n:
+        raise ValueError(f"array has {a.size} entries, the lattice has {n} nodes")
+    return a
+
+
+class Lattice:
+    """One lattice on one GPU: the device-side state of an LBM-C run."""
+
+    def __init__(self, lattice=D3Q15, length=(16, 16, 16), collision_type=BGK, forcing=False,
+                 macro_bc=False, micro_bc=False, tau=1.0, tau_mrt=None, c_smag=0.0,
+                 storage=FP32, quirks=0, device=0):
+        self.lattice = lattice
+        self.dim = 2 if lattice == D2Q9 else 3
+        self.q = 9 if lattice == D2Q9 else 15
+        L = list(length) + [1] * (3 - len(length))
+        self.length = tuple(int(v) for v in L[: self.dim])
+        self.n = int(np.prod(self.length))
+        self.storage = storage
+        d = Desc()
+        d.lattice = lattice
+        for k in range(3):
+            d.length[k] = int(L[k]) if k < self.dim else 1
+        d.collision_type = collision_type
+        d.forcing = int(bool(forcing))
+        d.macro_bc = int(bool(macro_bc))
+        d.micro_bc = int(bool(micro_bc))
+        d.tau = float(tau)
+        if tau_mrt is not None:
+            for k, v in enumerate(tau_mrt):
+                d.tau_mrt[k] = float(v)
+        d.c_smag = float(c_smag)
+        d.storage = storage
+        d.quirks = quirks
+        d.device = device
+        self.desc = d
+        self._h = ct.c_void_p()
+        st = _lib.lbmc_create(ct.byref(d), ct.byref(self._h))
+        if st != 0:
+            raise LbmcError(st, _lib.lbmc_last_error(None).decode())
+
+    # -- plumbing
+    def _check(self, st):
+        if st != 0:
+            raise LbmcError(st, _lib.lbmc_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            _lib.lbmc_destroy(self._h)
+            self._h = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- setup
+    def set_fields(self, geometry=None, macro_bc=None, micro_bc=None, force=None, u=None,
+                   rho=None):
+        n = self.n
+        g = _f64(geometry, n)
+        ma = _i32(macro_bc, n)
+        mi = _i32(micro_bc, n)
+        fo = [_f64(v, n) for v in force] if force is not None else None
+        uu = [_f64(v, n) for v in u] if u is not None else None
+        r = _f64(rho, n)
+        self._keep = (g, ma, mi, fo, uu, r)
+        self._check(_lib.lbmc_set_fields(
+            self._h,
+            _dptr(g) if g is not None else None,
+            ma.ctypes.data_as(ct.POINTER(ct.c_int)) if ma is not None else None,
+            mi.ctypes.data_as(ct.POINTER(ct.c_int)) if mi is not None else None,
+            _ptr_array(fo, 3) if fo is not None else None,
+            _ptr_array(uu, 3) if uu is not None else None,
+            _dptr(r) if r is not None else None))
+
+    def init_static(self):
+        self._check(_lib.lbmc_init_static(self._h))
+
+    def init_equilibrium(self, rho, u):
+        n = self.n
+        r = _f64(rho, n)
+        uu = [_f64(v, n) for v in u]
+        self._check(_lib.lbmc_init_equilibrium(self._h, _dptr(r), _ptr_array(uu, 3)))
+
+    # -- stepping
+    def step(self, t, store_macros=True):
+        self._check(_lib.lbmc_step(self._h, int(t), int(bool(store_macros))))
+
+    def run(self, t0, nsteps, store_every=1):
+        self._check(_lib.lbmc_run(self._h, int(t0), int(nsteps), int(store_every)))
+
+    def time_steps(self, t0, nsteps, store_every=0):
+        ms = ct.c_float()
+        self._check(_lib.lbmc_time_steps(self._h, int(t0), int(nsteps), int(store_every),
+                                         ct.byref(ms)))
+        return float(ms.value)
+
+    def sync(self):
+        self._check(_lib.lbmc_sync(self._h))
+
+    # -- read-back
+    def get_macros(self):
+        rho = np.empty(self.n)
+        u = [np.empty(self.n) for _ in range(self.dim)]
+        self._check(_lib.lbmc_get_macros(self._h, _dptr(rho), _ptr_array(u, 3)))
+        return rho, np.stack(u)
+
+    def get_node(self, idx):
+        rho = ct.c_double()
+        u = (ct.c_double * 3)()
+        self._check(_lib.lbmc_get_node(self._h, int(idx), ct.byref(rho), u))
+        return rho.value, np.array(u[: self.dim])
+
+    def get_f(self):
+        f = [np.empty(self.n) for _ in range(self.q)]
+        self._check(_lib.lbmc_get_f(self._h, _ptr_array(f, 15)))
+        return np.stack(f)
+
+    def set_f(self, f):
+        f = [_f64(v, self.n) for v in f]
+        self._check(_lib.lbmc_set_f(self._h, _ptr_array(f, 15)))
+
+    def node_types(self):
+        mask = np.empty(self.n, dtype=np.uint8)
+        table = np.zeros(256, dtype=np.uint16)
+        nc = ct.c_int()
+        self._check(_lib.lbmc_get_node_types(
+            self._h, mask.ctypes.data_as(ct.POINTER(ct.c_uint8)),
+            table.ctypes.data_as(ct.POINTER(ct.c_uint16)), ct.byref(nc)))
+        return mask, table[: nc.value].copy()
+
+    def links(self):
+        out = np.empty(self.n, dtype=np.uint16)
+        self._check(_lib.lbmc_get_links(self._h, out.ctypes.data_as(ct.POINTER(ct.c_uint16))))
+        return out
+
+    def kinetic_energy(self):
+        e = ct.c_double()
+        self._check(_lib.lbmc_kinetic_energy(self._h, ct.byref(e)))
+        return e.value
+
+
+def copy_bandwidth(nbytes, reps=20, device=0):
+    """Time a plain device copy of nbytes (read nbytes + write nbytes) reps times; ms total."""
+    ms = ct.c_float()
+    st = _lib.lbmc_copy_bandwidth(int(device), ct.c_size_t(int(nbytes)), int(reps), ct.byref(ms))
+    if st != 0:
+        raise LbmcError(st, _lib.lbmc_last_error(None).decode())
+    return float(ms.value)

@@ -1,0 +1,88 @@
+"""Seeded synthetic inputs for the benchmark configurations.
+
+  decaying_turbulence  periodic box, solenoidal random-phase velocity with
+                       E(k) ~ k^4 exp(-2 (k/k0)^2), scaled to a per-component
+                       rms velocity (the D3Q15 turbulence workloads)
+  channel              body-force channel with solid walls at y=0 and y=Ly-1
+  porous_discs         non-overlapping discs with radii in [rmin, rmax] placed
+                       until a solid fraction is reached, walls top/bottom,
+                       inlet/outlet columns kept clear (the D2Q9 porous case)
+
+All generators take an explicit seed and use numpy's PCG64 generator, so the
+fields (and the node masks built from them) are reproducible bit for bit.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def decaying_turbulence(L, k0=4.0, u_rms=0.05, seed=0):
+    """Returns (ux, uy, uz) float64 arrays of shape (L, L, L) (z, y, x order),
+    divergence-free in the spectral sense, with per-component rms u_rms."""
+    rng = np.random.default_rng(seed)
+    shape = (L, L, L)
+    k1 = np.fft.fftfreq(L, d=1.0 / L)
+    kr = np.fft.rfftfreq(L, d=1.0 / L)
+    kz, ky, kx = np.meshgrid(k1, k1, kr, indexing="ij")
+    k2 = kx * kx + ky * ky + kz * kz
+    k = np.sqrt(k2)
+    # random phases: transform of white noise in each component
+    comps = [np.fft.rfftn(rng.standard_normal(shape)) for _ in range(3)]
+    # solenoidal projection u_hat - k (k . u_hat)/|k|^2
+    with np.errstate(invalid="ignore", divide="ignore"):
+        dot = (kx * comps[0] + ky * comps[1] + kz * comps[2]) / np.where(k2 > 0, k2, 1.0)
+    comps = [comps[0] - kx * dot, comps[1] - ky * dot, comps[2] - kz * dot]
+    # shell spectrum E(k) ~ k^4 exp(-2(k/k0)^2): amplitude ~ sqrt(E(k)/k^2)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        amp = np.where(k > 0, np.sqrt(k ** 4 * np.exp(-2.0 * (k / k0) ** 2) / np.where(k2 > 0, k2, 1.0)), 0.0)
+    u = [np.fft.irfftn(c * amp, s=shape) for c in comps]
+    scale = u_rms / np.sqrt(np.mean(u[0] ** 2 + u[1] ** 2 + u[2] ** 2) / 3.0)
+    return tuple(np.ascontiguousarray(v * scale) for v in u)
+
+
+def energy_spectrum(ux, uy, uz):
+    """Shell-averaged E(k) of a periodic (L, L, L) field; k = 0..L/2."""
+    L = ux.shape[0]
+    k1 = np.fft.fftfreq(L, d=1.0 / L)
+    kz, ky, kx = np.meshgrid(k1, k1, k1, indexing="ij")
+    kk = np.rint(np.sqrt(kx * kx + ky * ky + kz * kz)).astype(int)
+    e = np.zeros(L // 2 + 1)
+    for c in (ux, uy, uz):
+        p = np.abs(np.fft.fftn(c) / c.size) ** 2
+        e += np.bincount(kk.reshape(-1), weights=0.5 * p.reshape(-1), minlength=kk.max() + 1)[: L // 2 + 1]
+    return e
+
+
+def channel(Lx, Ly):
+    """Porosity field of a channel: solid rows y=0 and y=Ly-1."""
+    geo = np.zeros((Ly, Lx))
+    geo[0, :] = 1.0
+    geo[-1, :] = 1.0
+    return geo
+
+
+def porous_discs(Lx, Ly, rmin=8, rmax=24, solid_fraction=0.30, seed=0, margin=None,
+                 max_tries=200000):
+    """Porosity field with seeded non-overlapping discs (radius uniform in
+    [rmin, rmax]) added until the solid fraction (walls included) reaches the
+    target.  Discs stay `margin` columns (default rmax) clear of the inlet and
+    outlet and one node clear of each other and of the walls."""
+    rng = np.random.default_rng(seed)
+    margin = rmax if margin is None else margin
+    geo = channel(Lx, Ly)
+    discs = []
+    yy, xx = np.ogrid[0:Ly, 0:Lx]
+    tries = 0
+    while geo.mean() < solid_fraction and tries < max_tries:
+        tries += 1
+        r = rng.uniform(rmin, rmax)
+        cx = rng.uniform(margin + r, Lx - 1 - margin - r)
+        cy = rng.uniform(1 + r + 1, Ly - 2 - r - 1)
+        if any((cx - x) ** 2 + (cy - y) ** 2 < (r + s + 1) ** 2 for x, y, s in discs):
+            continue
+        discs.append((cx, cy, r))
+        x0, x1 = int(cx - r - 1), int(cx + r + 2)
+        y0, y1 = int(cy - r - 1), int(cy + r + 2)
+        sub = (xx[:, x0:x1] - cx) ** 2 + (yy[y0:y1, :] - cy) ** 2 <= r * r
+        geo[y0:y1, x0:x1][sub] = 1.0
+    return geo, discs
