@@ -55,7 +55,8 @@ struct lbmc_ctx {
     KernelKey key{};
 
     // launch shape
-    int xcap = 256, rows_per_pass = 1, rows_per_block = 1, xseg = 256, nseg = 1, grid = 1;
+    int grid = 1, remap = 0;
+    FastDiv dx{}, dy{};
 
     std::string err;
 };
@@ -149,12 +150,12 @@ void fill_args(const lbmc_ctx *c, StepArgs<C> &a, long long t, int store)
     a.rho = (C *)c->rho;
     a.sfbuf = (const C *)c->sfbuf;
     a.sfidx = c->sfidx;
-    a.xcap = c->xcap;
-    a.rows_per_pass = c->rows_per_pass;
-    a.rows_per_block = c->rows_per_block;
-    a.nrows = c->L[1] * c->L[2];
-    a.xseg = c->xseg;
-    a.nseg = c->nseg;
+    a.nnodes = (int)c->N;
+    a.mx = c->dx.m;
+    a.lx = c->dx.l;
+    a.my = c->dy.m;
+    a.ly = c->dy.l;
+    a.remap = c->remap;
     a.tau = (C)c->desc.tau;
     a.c_smag = (C)c->desc.c_smag;
     for (int i = 0; i < 15; i++) a.tau_mrt[i] = (C)c->desc.tau_mrt[i];
@@ -195,28 +196,17 @@ lbmc_status launch_step(lbmc_ctx *c, long long t, int store)
     return LBMC_OK;
 }
 
-// Launch shape: a block of 256 threads covers whole x-rows when Lx <= 256
-// (256/Lx rows at a time), else x-segments of xseg nodes.  The grid has one
-// block per (row group, segment).
+// Launch shape: one thread per node, 256-thread blocks.  With a grid that is
+// a multiple of 8 the blocks can be reordered so that each XCD works on a
+// contiguous range of nodes (LBMC_XCD_REMAP=1 turns it on).
 void choose_shape(lbmc_ctx *c)
 {
-    const int Lx = c->L[0];
-    const int nrows = c->L[1] * c->L[2];
-    c->xcap = std::min(Lx, kBlock);
-    c->rows_per_pass = kBlock / c->xcap;
-    c->rows_per_block = c->rows_per_pass;
-    c->xseg = Lx <= kBlock ? Lx : kBlock;
-    if (const char *env = getenv("LBMC_XSEG")) {
-        const int v = atoi(env);
-        if (v >= c->xcap && v % c->xcap == 0) c->xseg = v;
-    }
-    if (const char *env = getenv("LBMC_ROWS_PER_BLOCK")) {
-        const int v = atoi(env);
-        if (v >= 1) c->rows_per_block = c->rows_per_pass * v;
-    }
-    c->nseg = (Lx + c->xseg - 1) / c->xseg;
-    const long long groups = (nrows + c->rows_per_block - 1) / c->rows_per_block;
-    c->grid = (int)(groups * c->nseg);
+    c->grid = (int)((c->N + kBlock - 1) / kBlock);
+    c->dx = make_fastdiv(c->L[0]);
+    c->dy = make_fastdiv(c->L[1]);
+    c->remap = 0;
+    if (const char *env = getenv("LBMC_XCD_REMAP"))
+        if (atoi(env) != 0 && c->grid % 8 == 0) c->remap = 1;
 }
 
 }  // namespace

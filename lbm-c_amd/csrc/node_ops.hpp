@@ -152,6 +152,65 @@ __device__ __forceinline__ C smagorinsky_tau(C tau, C rho, C c, const C *f, cons
                            tau);
 }
 
+// ---- FP32-compute forms.  The FP32/FP16S/BF16S paths compare to the FP64
+// oracle within a tolerance, so they may regroup the arithmetic; these forms
+// are algebraically identical to the reference's and cheaper:
+//   * equilibrium: opposite directions share the even part,
+//     feq_i = rho w (1 - usq + (eu)^2/2) +- rho w eu;
+//   * Smagorinsky: sqrt(2 x^2) = sqrt(2)|x| and the symmetric Pi_ab = Pi_ba,
+//     so Qhat = sqrt(2) (sum_a |Pi_aa| + 2 sum_{a<b} |Pi_ab|), and
+//     2 sqrt(2) C^2 (9/rho) Qhat = 36 C^2 S / rho with S the bracket above;
+//     tau + (sqrt(tau^2 + A) - tau)/2 = (tau + sqrt(tau^2 + A))/2.
+
+template <class L, class C>
+__device__ __forceinline__ void equilibrium_paired(C rho, const C *u, C *feq)
+{
+    const C base = C(1) - u_square<L>(u);
+    sfor<0, L::Q>([&](auto ii) {
+        constexpr int i = decltype(ii)::value;
+        constexpr int o = L::opp[i];
+        if constexpr (i == 0) {
+            feq[0] = rho * C(L::w[0]) * base;
+        } else if constexpr (i < o) {
+            const C eu = edot<L, i>(u) * C(3);
+            const C rw = rho * C(L::w[i]);
+            const C even = rw * (base + C(0.5) * eu * eu);
+            const C odd = rw * eu;
+            feq[i] = even + odd;
+            feq[o] = even - odd;
+        }
+    });
+}
+
+template <class L, class C>
+__device__ __forceinline__ C smagorinsky_tau_fast(C tau, C inv_rho, C c, const C *f, const C *feq)
+{
+    constexpr int D = L::D;
+    C d[L::Q];
+    sfor<0, L::Q>([&](auto qq) {
+        constexpr int q = decltype(qq)::value;
+        d[q] = f[q] - feq[q];
+    });
+    C sdiag = C(0), soff = C(0);
+    sfor<0, D>([&](auto aa) {
+        constexpr int a = decltype(aa)::value;
+        sfor<a, D>([&](auto bb) {
+            constexpr int b = decltype(bb)::value;
+            C s = C(0);
+            sfor<0, L::Q>([&](auto qq) {
+                constexpr int q = decltype(qq)::value;
+                constexpr int p = e_of<L>(a, q) * e_of<L>(b, q);
+                if constexpr (p > 0) s += d[q];
+                else if constexpr (p < 0) s -= d[q];
+            });
+            if constexpr (a == b) sdiag += fabs(s);
+            else soff += fabs(s);
+        });
+    });
+    const C A = C(36) * (c * c) * (sdiag + C(2) * soff) * inv_rho;
+    return C(0.5) * (tau + sqrt(tau * tau + A));
+}
+
 // moment-space equilibria, collision.cu:465-503
 template <class L, class C>
 __device__ __forceinline__ void moment_equilibrium(C rho, const C *u, C qc, C *meq)

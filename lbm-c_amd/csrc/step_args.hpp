@@ -8,6 +8,20 @@ namespace lbmc {
 // threads per block of the step kernels
 constexpr int kBlock = 256;
 
+// Division by a fixed divisor d >= 1 for numerators n < 2^31:
+// l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1, n / d = (umulhi(n, m) + n) >> l.
+struct FastDiv {
+    unsigned m;
+    int l;
+};
+inline FastDiv make_fastdiv(int d)
+{
+    int l = 0;
+    while ((1LL << l) < d) l++;
+    const unsigned long long m = ((1ULL << 32) * ((1ULL << l) - (unsigned long long)d)) / (unsigned long long)d + 1;
+    return FastDiv{(unsigned)m, l};
+}
+
 // which fused kernel variant a launch needs
 struct KernelKey {
     int lattice;  // 0 D2Q9, 1 D3Q15
@@ -38,12 +52,10 @@ struct StepArgs {
     const int *sfidx;        // per node: row of sfbuf (sf nodes only)
     int L[3];
     int s[3];                // t mod L[d]: the index-shift of this call
-    int xcap;                // threads along x per row pass
-    int rows_per_pass;       // rows processed concurrently by one block
-    int rows_per_block;
-    int nrows;               // Ly * Lz
-    int xseg;                // x extent of one block (multiple of xcap)
-    int nseg;                // ceil(Lx / xseg): blocks per row group
+    int nnodes;              // Lx * Ly * Lz
+    unsigned mx, my;         // magic multipliers: n / Lx and row / Ly
+    int lx, ly;              //   (q = (umulhi(n, m) + n) >> l, n < 2^31)
+    int remap;               // XCD-contiguous block order (grid % 8 == 0)
     C tau;
     C c_smag;
     C tau_mrt[15];

@@ -12,11 +12,11 @@
 //      no ghost layers: each slot is owned by exactly one node per call;
 //   5. rho and u are stored when asked (solver.cu:125-134).
 //
-// Work decomposition: a block owns whole x-rows (or x-segments of long rows);
-// for a fixed direction the slots a row touches are one x-row rotated by
-// t*e_x, so a block that covers a full row reads and writes exactly the cache
-// lines of that row.  No __syncthreads and no LDS are needed: nodes are
-// independent within a call.
+// Work decomposition: one thread per node in slot order.  For a fixed
+// direction a wave's 64 nodes read 64 consecutive slots of the source row
+// (rotated by t*e_x), so the shift costs at most one extra cache line per
+// row.  No __syncthreads and no LDS are needed: nodes are independent within
+// a call.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -28,224 +28,264 @@
 
 namespace lbmc {
 
+// One node of iterate_kernel after the gather (solver.cu:95-134): v[] holds
+// the node's pulled populations as stored (f_i, or g_i = f_i - w_i for the
+// shifted formats) converted to the compute type; on return v[] holds the
+// post-collision values in the same form, ready to be written back to the
+// same slots.  Stores rho/u when the call asks for it.
+template <class L, class St, int FAM, bool GUO, bool SMAG, bool MFIX>
+__device__ __forceinline__ void node_update(const StepArgs<typename St::C> &a, int n,
+                                            typename St::C (&v)[L::Q])
+{
+    using C = typename St::C;
+    constexpr int Q = L::Q;
+    constexpr int D = L::D;
+    const C qc = (a.quirks & QK_FIX_D3Q15_MEQ) ? C(-7.0 / 3.0) : C(-2);
+    const bool zhfix = (a.quirks & QK_FIX_D3Q15_ZH) != 0;
+
+    // ---- moments, in the reference's summation order
+    C f[Q];
+    C rho = C(0);
+    C j[3] = {C(0), C(0), C(0)};
+    bool jfirst[3] = {true, true, true};
+    sfor<0, Q>([&](auto ii) {
+        constexpr int i = decltype(ii)::value;
+        rho = (i == 0) ? v[i] : rho + v[i];
+        sfor<0, D>([&](auto dd) {
+            constexpr int d = decltype(dd)::value;
+            constexpr int e = e_of<L>(d, i);
+            if constexpr (e != 0) {
+                if (jfirst[d]) j[d] = (e > 0) ? v[i] : -v[i];
+                else j[d] = (e > 0) ? j[d] + v[i] : j[d] - v[i];
+                jfirst[d] = false;
+            }
+        });
+        if constexpr (St::shifted) f[i] = v[i] + C(L::w[i]);
+        else f[i] = v[i];
+    });
+    if constexpr (St::shifted) rho = C(1) + rho;
+    // FP32 compute regroups the arithmetic (one reciprocal of rho, paired
+    // equilibria, the closed-form Smagorinsky norm); FP64 keeps the
+    // reference's operation order bit for bit.
+    constexpr bool FAST = std::is_same<C, float>::value;
+    C u[3];
+    if constexpr (FAST) {
+        const C ir = C(1) / rho;
+        sfor<0, D>([&](auto dd) { u[decltype(dd)::value] = j[decltype(dd)::value] * ir; });
+    } else {
+        sfor<0, D>([&](auto dd) {
+            constexpr int d = decltype(dd)::value;
+            u[d] = j[d] / rho;
+        });
+    }
+
+    // ---- node class
+    const unsigned m = a.mask[n];
+    const uint16_t desc = m ? a.cls[m] : (uint16_t)0;
+
+    if (desc & CLS_SOLID) {
+        // bounceback, collision.cu:423-443
+        C t[Q];
+        sfor<0, Q>([&](auto ii) {
+            constexpr int i = decltype(ii)::value;
+            t[i] = f[L::opp[i]];
+        });
+        sfor<0, Q>([&](auto ii) {
+            constexpr int i = decltype(ii)::value;
+            f[i] = t[i];
+        });
+        rho = C(0);
+        sfor<0, D>([&](auto dd) { u[decltype(dd)::value] = C(0); });
+    } else {
+        C B = C(0);
+        if (desc) {
+            if (desc & CLS_POROUS) B = a.porosity[n];
+            const int mc = cls_macro(desc);
+            if (mc) {
+                // presc_*, macro_bc.cu:19-125
+                const unsigned ps = macro_set(mc);
+                if (ps & 1u) rho = a.rho[n];
+                sfor<0, D>([&](auto dd) {
+                    constexpr int d = decltype(dd)::value;
+                    if (ps & (2u << d)) u[d] = a.u[d][n];
+                });
+            }
+            const int mi = cls_micro(desc);
+            if (mi) {
+                const C *nb = a.sfidx ? a.sfbuf + (size_t)a.sfidx[n] * Q : nullptr;
+                apply_micro<L>(mi, f, rho, u, nb, zhfix);
+            }
+        }
+
+        C F[3] = {C(0), C(0), C(0)};
+        if constexpr (GUO) {
+            if (a.force[0]) {
+                sfor<0, D>([&](auto dd) {
+                    constexpr int d = decltype(dd)::value;
+                    F[d] = a.force[d][n];
+                });
+            } else {
+                sfor<0, D>([&](auto dd) {
+                    constexpr int d = decltype(dd)::value;
+                    F[d] = a.F[d];
+                });
+            }
+        }
+
+        if constexpr (FAM == 0) {
+            // bgk / bgk_guo / bgk_ntpor / bgk_ntpor_guo, collision.cu:49-190
+            C ft[Q];
+            C tau = a.tau;
+            if constexpr (GUO) {
+                guo_shift<L>(rho, F, u);
+                guo_source<L>(tau, u, F, ft);
+            }
+            C feq[Q];
+            if constexpr (FAST) equilibrium_paired<L>(rho, u, feq);
+            else equilibrium<L>(rho, u, feq);
+            if constexpr (SMAG) {
+                if (a.c_smag > C(0)) {
+                    if constexpr (FAST) tau = smagorinsky_tau_fast<L>(tau, C(1) / rho, a.c_smag, f, feq);
+                    else tau = smagorinsky_tau<L>(tau, rho, a.c_smag, f, feq);
+                }
+            }
+            const C omega = C(1.0) / tau;
+            if (a.porous) {
+                const C ob = C(1) - B;
+                C t[Q];
+                sfor<0, Q>([&](auto ii) {
+                    constexpr int i = decltype(ii)::value;
+                    const C cb = omega * (f[i] - feq[i]);
+                    const C cs = f[L::opp[i]] - f[i];
+                    C r = f[i] - ob * cb + B * cs;
+                    if constexpr (GUO) r = r + ob * ft[i];
+                    t[i] = r;
+                });
+                sfor<0, Q>([&](auto ii) {
+                    constexpr int i = decltype(ii)::value;
+                    f[i] = t[i];
+                });
+            } else {
+                sfor<0, Q>([&](auto ii) {
+                    constexpr int i = decltype(ii)::value;
+                    C r = f[i] - omega * (f[i] - feq[i]);
+                    if constexpr (GUO) r = r + ft[i];
+                    f[i] = r;
+                });
+            }
+        } else {
+            // mrt / mrt_guo / mrt_ntpor / mrt_ntpor_guo, collision.cu:192-421
+            C ft[Q];
+            if constexpr (GUO) {
+                guo_shift<L>(rho, F, u);
+                guo_source<L>(a.tau, u, F, ft);
+            }
+            C meq[Q], mm[Q];
+            moment_equilibrium<L>(rho, u, qc, meq);
+            mrt_moments<L>(f, meq, a.tau_mrt, mm);
+            if (a.porous) {
+                // in place: f[opp[i]] is already post-collision when
+                // opp[i] < i (collision.cu:331-343, :406-419)
+                const C ob = C(1) - B;
+                sfor<0, Q>([&](auto ii) {
+                    constexpr int i = decltype(ii)::value;
+                    const C acc = minv_row<L, MFIX, i>(mm);
+                    const C cs = f[L::opp[i]] - f[i];
+                    C r = f[i] - ob * acc;
+                    if constexpr (GUO) r = r + ob * ft[i];
+                    f[i] = r + B * cs;
+                });
+            } else {
+                sfor<0, Q>([&](auto ii) {
+                    constexpr int i = decltype(ii)::value;
+                    C r = f[i] - minv_row<L, MFIX, i>(mm);
+                    if constexpr (GUO) r = r + ft[i];
+                    f[i] = r;
+                });
+            }
+        }
+    }
+
+    sfor<0, Q>([&](auto ii) {
+        constexpr int i = decltype(ii)::value;
+        if constexpr (St::shifted) v[i] = f[i] - C(L::w[i]);
+        else v[i] = f[i];
+    });
+    if (a.store) {
+        sfor<0, D>([&](auto dd) {
+            constexpr int d = decltype(dd)::value;
+            a.u[d][n] = u[d];
+        });
+        a.rho[n] = rho;
+    }
+}
+
+__device__ __forceinline__ int fast_div(int n, unsigned m, int l)
+{
+    return (int)((__umulhi((unsigned)n, m) + (unsigned)n) >> l);
+}
+
+// One thread per node, nodes in slot order (x fastest).  For a fixed
+// direction, consecutive nodes read consecutive slots of the source row
+// (rotated by t*e_x), so every wave's accesses are contiguous apart from
+// at most one wrap point.
 template <class L, class St, int FAM, bool GUO, bool SMAG, bool MFIX>
 __global__ __launch_bounds__(kBlock) void step_kernel(const StepArgs<typename St::C> a)
 {
     using C = typename St::C;
     using S = typename St::S;
     constexpr int Q = L::Q;
-    constexpr int D = L::D;
 
-    const int Lx = a.L[0], Ly = a.L[1], Lz = a.L[2];
-    const int sx = a.s[0], sy = a.s[1], sz = a.s[2];
-    const int tx = (int)threadIdx.x % a.xcap;
-    const int ty = (int)threadIdx.x / a.xcap;
-    if (ty >= a.rows_per_pass) return;
-    const int seg = (int)blockIdx.x % a.nseg;
-    const int row0 = ((int)blockIdx.x / a.nseg) * a.rows_per_block;
-    const int row1 = min(row0 + a.rows_per_block, a.nrows);
-    const int xbeg = seg * a.xseg;
-    const int xend = min(Lx, xbeg + a.xseg);
-    S *const fb = (S *)a.f;
-
-    const C qc = (a.quirks & QK_FIX_D3Q15_MEQ) ? C(-7.0 / 3.0) : C(-2);
-    const bool zhfix = (a.quirks & QK_FIX_D3Q15_ZH) != 0;
-
-    for (int row = row0 + ty; row < row1; row += a.rows_per_pass) {
-        const int y = row % Ly;
-        const int z = row / Ly;
-        // source rows for e_y, e_z in {-1, 0, 1} (index e+1)
-        int yy[3], zz[3];
-        yy[1] = y;
-        yy[2] = y - sy;
-        if (yy[2] < 0) yy[2] += Ly;
-        yy[0] = y + sy;
-        if (yy[0] >= Ly) yy[0] -= Ly;
-        zz[1] = z;
-        zz[2] = z - sz;
-        if (zz[2] < 0) zz[2] += Lz;
-        zz[0] = z + sz;
-        if (zz[0] >= Lz) zz[0] -= Lz;
-
-        for (int x = xbeg + tx; x < xend; x += a.xcap) {
-            int xx[3];
-            xx[1] = x;
-            xx[2] = x - sx;
-            if (xx[2] < 0) xx[2] += Lx;
-            xx[0] = x + sx;
-            if (xx[0] >= Lx) xx[0] -= Lx;
-            const int n = row * Lx + x;
-
-            // ---- gather and moments
-            int idx[Q];
-            C f[Q];
-            C rho = C(0);
-            C j[3] = {C(0), C(0), C(0)};
-            bool jfirst[3] = {true, true, true};
-            sfor<0, Q>([&](auto ii) {
-                constexpr int i = decltype(ii)::value;
-                idx[i] = (zz[L::ez[i] + 1] * Ly + yy[L::ey[i] + 1]) * Lx + xx[L::ex[i] + 1];
-                const C v = St::load(fb[(size_t)i * a.npad + idx[i]]);
-                rho = (i == 0) ? v : rho + v;
-                sfor<0, D>([&](auto dd) {
-                    constexpr int d = decltype(dd)::value;
-                    constexpr int e = e_of<L>(d, i);
-                    if constexpr (e != 0) {
-                        if (jfirst[d]) j[d] = (e > 0) ? v : -v;
-                        else j[d] = (e > 0) ? j[d] + v : j[d] - v;
-                        jfirst[d] = false;
-                    }
-                });
-                if constexpr (St::shifted) f[i] = v + C(L::w[i]);
-                else f[i] = v;
-            });
-            if constexpr (St::shifted) rho = C(1) + rho;
-            C u[3];
-            sfor<0, D>([&](auto dd) {
-                constexpr int d = decltype(dd)::value;
-                u[d] = j[d] / rho;
-            });
-
-            // ---- node class
-            const unsigned m = a.mask[n];
-            const uint16_t desc = m ? a.cls[m] : (uint16_t)0;
-
-            if (desc & CLS_SOLID) {
-                // bounceback, collision.cu:423-443
-                C t[Q];
-                sfor<0, Q>([&](auto ii) {
-                    constexpr int i = decltype(ii)::value;
-                    t[i] = f[L::opp[i]];
-                });
-                sfor<0, Q>([&](auto ii) {
-                    constexpr int i = decltype(ii)::value;
-                    f[i] = t[i];
-                });
-                rho = C(0);
-                sfor<0, D>([&](auto dd) { u[decltype(dd)::value] = C(0); });
-            } else {
-                C B = C(0);
-                if (desc) {
-                    if (desc & CLS_POROUS) B = a.porosity[n];
-                    const int mc = cls_macro(desc);
-                    if (mc) {
-                        // presc_*, macro_bc.cu:19-125
-                        const unsigned ps = macro_set(mc);
-                        if (ps & 1u) rho = a.rho[n];
-                        sfor<0, D>([&](auto dd) {
-                            constexpr int d = decltype(dd)::value;
-                            if (ps & (2u << d)) u[d] = a.u[d][n];
-                        });
-                    }
-                    const int mi = cls_micro(desc);
-                    if (mi) {
-                        const C *nb = a.sfidx ? a.sfbuf + (size_t)a.sfidx[n] * Q : nullptr;
-                        apply_micro<L>(mi, f, rho, u, nb, zhfix);
-                    }
-                }
-
-                C F[3] = {C(0), C(0), C(0)};
-                if constexpr (GUO) {
-                    if (a.force[0]) {
-                        sfor<0, D>([&](auto dd) {
-                            constexpr int d = decltype(dd)::value;
-                            F[d] = a.force[d][n];
-                        });
-                    } else {
-                        sfor<0, D>([&](auto dd) {
-                            constexpr int d = decltype(dd)::value;
-                            F[d] = a.F[d];
-                        });
-                    }
-                }
-
-                if constexpr (FAM == 0) {
-                    // bgk / bgk_guo / bgk_ntpor / bgk_ntpor_guo, collision.cu:49-190
-                    C ft[Q];
-                    C tau = a.tau;
-                    if constexpr (GUO) {
-                        guo_shift<L>(rho, F, u);
-                        guo_source<L>(tau, u, F, ft);
-                    }
-                    C feq[Q];
-                    equilibrium<L>(rho, u, feq);
-                    if constexpr (SMAG) {
-                        if (a.c_smag > C(0)) tau = smagorinsky_tau<L>(tau, rho, a.c_smag, f, feq);
-                    }
-                    const C omega = C(1.0) / tau;
-                    if (a.porous) {
-                        const C ob = C(1) - B;
-                        C t[Q];
-                        sfor<0, Q>([&](auto ii) {
-                            constexpr int i = decltype(ii)::value;
-                            const C cb = omega * (f[i] - feq[i]);
-                            const C cs = f[L::opp[i]] - f[i];
-                            C r = f[i] - ob * cb + B * cs;
-                            if constexpr (GUO) r = r + ob * ft[i];
-                            t[i] = r;
-                        });
-                        sfor<0, Q>([&](auto ii) {
-                            constexpr int i = decltype(ii)::value;
-                            f[i] = t[i];
-                        });
-                    } else {
-                        sfor<0, Q>([&](auto ii) {
-                            constexpr int i = decltype(ii)::value;
-                            C r = f[i] - omega * (f[i] - feq[i]);
-                            if constexpr (GUO) r = r + ft[i];
-                            f[i] = r;
-                        });
-                    }
-                } else {
-                    // mrt / mrt_guo / mrt_ntpor / mrt_ntpor_guo, collision.cu:192-421
-                    C ft[Q];
-                    if constexpr (GUO) {
-                        guo_shift<L>(rho, F, u);
-                        guo_source<L>(a.tau, u, F, ft);
-                    }
-                    C meq[Q], mm[Q];
-                    moment_equilibrium<L>(rho, u, qc, meq);
-                    mrt_moments<L>(f, meq, a.tau_mrt, mm);
-                    if (a.porous) {
-                        // in place: f[opp[i]] is already post-collision when
-                        // opp[i] < i (collision.cu:331-343, :406-419)
-                        const C ob = C(1) - B;
-                        sfor<0, Q>([&](auto ii) {
-                            constexpr int i = decltype(ii)::value;
-                            const C acc = minv_row<L, MFIX, i>(mm);
-                            const C cs = f[L::opp[i]] - f[i];
-                            C r = f[i] - ob * acc;
-                            if constexpr (GUO) r = r + ob * ft[i];
-                            f[i] = r + B * cs;
-                        });
-                    } else {
-                        sfor<0, Q>([&](auto ii) {
-                            constexpr int i = decltype(ii)::value;
-                            C r = f[i] - minv_row<L, MFIX, i>(mm);
-                            if constexpr (GUO) r = r + ft[i];
-                            f[i] = r;
-                        });
-                    }
-                }
-            }
-
-            // ---- scatter to the same slots
-            sfor<0, Q>([&](auto ii) {
-                constexpr int i = decltype(ii)::value;
-                C v = f[i];
-                if constexpr (St::shifted) v = v - C(L::w[i]);
-                fb[(size_t)i * a.npad + idx[i]] = St::store(v);
-            });
-
-            if (a.store) {
-                sfor<0, D>([&](auto dd) {
-                    constexpr int d = decltype(dd)::value;
-                    a.u[d][n] = u[d];
-                });
-                a.rho[n] = rho;
-            }
-        }
+    int b = (int)blockIdx.x;
+    if (a.remap) {
+        // blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+        // contiguous range of nodes so that the lines shared by neighbouring
+        // blocks sit in one L2
+        const int g8 = (int)gridDim.x >> 3;
+        b = (b & 7) * g8 + (b >> 3);
     }
+    const int n = b * kBlock + (int)threadIdx.x;
+    if (n >= a.nnodes) return;
+    const int Lx = a.L[0], Ly = a.L[1], Lz = a.L[2];
+    const int row = fast_div(n, a.mx, a.lx);
+    const int x = n - row * Lx;
+    const int z = fast_div(row, a.my, a.ly);
+    const int y = row - z * Ly;
+
+    // source coordinates for e in {-1, 0, 1} (index e+1), solver.cu:77-92
+    int xx[3], yy[3], zz[3];
+    xx[1] = x;
+    xx[2] = x - a.s[0];
+    if (xx[2] < 0) xx[2] += Lx;
+    xx[0] = x + a.s[0];
+    if (xx[0] >= Lx) xx[0] -= Lx;
+    yy[1] = y;
+    yy[2] = y - a.s[1];
+    if (yy[2] < 0) yy[2] += Ly;
+    yy[0] = y + a.s[1];
+    if (yy[0] >= Ly) yy[0] -= Ly;
+    zz[1] = z;
+    zz[2] = z - a.s[2];
+    if (zz[2] < 0) zz[2] += Lz;
+    zz[0] = z + a.s[2];
+    if (zz[0] >= Lz) zz[0] -= Lz;
+
+    S *const fb = (S *)a.f;
+    C v[Q];
+    sfor<0, Q>([&](auto ii) {
+        constexpr int i = decltype(ii)::value;
+        const int id = (zz[L::ez[i] + 1] * Ly + yy[L::ey[i] + 1]) * Lx + xx[L::ex[i] + 1];
+        v[i] = St::load(fb[(size_t)i * a.npad + id]);
+    });
+
+    node_update<L, St, FAM, GUO, SMAG, MFIX>(a, n, v);
+
+    // back to the same slots, solver.cu:119-123
+    sfor<0, Q>([&](auto ii) {
+        constexpr int i = decltype(ii)::value;
+        const int id = (zz[L::ez[i] + 1] * Ly + yy[L::ey[i] + 1]) * Lx + xx[L::ex[i] + 1];
+        fb[(size_t)i * a.npad + id] = St::store(v[i]);
+    });
 }
 
 // Gathers the post-stream populations of the neighbours of the sf nodes
