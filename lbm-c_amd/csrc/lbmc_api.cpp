@@ -150,6 +150,17 @@ void fill_args(const lbmc_ctx *c, StepArgs<C> &a, long long t, int store)
     a.rho = (C *)c->rho;
     a.sfbuf = (const C *)c->sfbuf;
     a.sfidx = c->sfidx;
+    {
+        long long stride = 1;
+        for (int d = 0; d < 3; d++) {
+            const long long L = c->L[d], sd = a.s[d];
+            a.dp[d] = (int)(-sd * stride);
+            a.dpw[d] = (int)((L - sd) * stride);
+            a.dm[d] = (int)(sd * stride);
+            a.dmw[d] = (int)((sd - L) * stride);
+            stride *= L;
+        }
+    }
     a.nnodes = (int)c->N;
     a.mx = c->dx.m;
     a.lx = c->dx.l;
@@ -245,6 +256,10 @@ lbmc_status lbmc_create(const lbmc_desc *desc, lbmc_ctx **out)
     for (int d = 0; d < 3; d++) c->L[d] = d < D ? desc->length[d] : 1;
     c->N = N;
     c->npad = (N + 63) / 64 * 64;
+    // tuning hook: extra elements between direction arrays (staggers their
+    // base addresses across HBM channels)
+    if (const char *env = getenv("LBMC_PAD"))
+        if (atoi(env) > 0) c->npad += (atoi(env) + 63) / 64 * 64;
     c->storage = desc->storage;
     c->sbytes = desc->storage == LBMC_FP64 ? 8 : desc->storage == LBMC_FP32 ? 4 : 2;
     c->cbytes = desc->storage == LBMC_FP64 ? 8 : 4;
@@ -253,6 +268,10 @@ lbmc_status lbmc_create(const lbmc_desc *desc, lbmc_ctx **out)
     c->key.guo = desc->forcing ? 1 : 0;
     c->key.smag = (c->key.family == 0 && desc->c_smag > 0) ? 1 : 0;
     c->key.mfix = (desc->lattice == LBMC_D3Q15 && (desc->quirks & LBMC_FIX_D3Q15_MINV)) ? 1 : 0;
+    c->key.wide = (unsigned long long)c->npad * c->sbytes >= (1ULL << 32) ? 1 : 0;
+    // test hook: run the 64-bit-offset kernels on a small lattice
+    if (const char *env = getenv("LBMC_FORCE_WIDE"))
+        if (atoi(env) != 0) c->key.wide = 1;
     choose_shape(c);
 
     auto bail = [&](lbmc_status st) {

@@ -15,6 +15,14 @@
 
 namespace lbmc {
 
+// 1/x: FP32 uses the hardware reciprocal (1 ulp); FP64 divides exactly
+template <class C>
+__device__ __forceinline__ C recip(C x)
+{
+    if constexpr (sizeof(C) == 4) return __builtin_amdgcn_rcpf(x);
+    else return C(1) / x;
+}
+
 template <int B, int E, class F>
 __device__ __forceinline__ void sfor(F &&fn)
 {

@@ -29,6 +29,7 @@ struct KernelKey {
     int guo;      // Guo body force
     int smag;     // Smagorinsky (BGK family only)
     int mfix;     // corrected D3Q15 M^-1 (MRT family only)
+    int wide;     // a direction array spans 4 GiB or more: 64-bit offsets
 };
 
 enum : unsigned {
@@ -56,6 +57,11 @@ struct StepArgs {
     unsigned mx, my;         // magic multipliers: n / Lx and row / Ly
     int lx, ly;              //   (q = (umulhi(n, m) + n) >> l, n < 2^31)
     int remap;               // XCD-contiguous block order (grid % 8 == 0)
+    // slot offsets of this call relative to the node index: for e = +1 along
+    // an axis the slot is the node shifted by -s (or by L - s once wrapped),
+    // for e = -1 by +s (or s - L); scaled by the axis stride
+    int dp[3], dpw[3];       // e = +1: -s*stride, (L - s)*stride
+    int dm[3], dmw[3];       // e = -1: +s*stride, (s - L)*stride
     C tau;
     C c_smag;
     C tau_mrt[15];

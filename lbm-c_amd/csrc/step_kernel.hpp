@@ -21,6 +21,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "fast_bgk.hpp"
 #include "lattice.hpp"
 #include "node_ops.hpp"
 #include "step_args.hpp"
@@ -42,6 +43,32 @@ __device__ __forceinline__ void node_update(const StepArgs<typename St::C> &a, i
     constexpr int D = L::D;
     const C qc = (a.quirks & QK_FIX_D3Q15_MEQ) ? C(-7.0 / 3.0) : C(-2);
     const bool zhfix = (a.quirks & QK_FIX_D3Q15_ZH) != 0;
+
+    // FP32 compute: plain fluid nodes of the BGK family take the regrouped
+    // update (fast_bgk.hpp); everything else follows the reference's order.
+    constexpr bool FAST = std::is_same<C, float>::value;
+    const unsigned m = a.mask[n];
+    if constexpr (FAST && FAM == 0) {
+        if (m == 0) {
+            C F[3] = {C(0), C(0), C(0)};
+            if constexpr (GUO) {
+                sfor<0, D>([&](auto dd) {
+                    constexpr int d = decltype(dd)::value;
+                    F[d] = a.force[0] ? a.force[d][n] : a.F[d];
+                });
+            }
+            C rho, u[3];
+            fluid_bgk_fast<L, St::shifted, GUO, SMAG>(v, a.tau, a.c_smag, F, rho, u);
+            if (a.store) {
+                sfor<0, D>([&](auto dd) {
+                    constexpr int d = decltype(dd)::value;
+                    a.u[d][n] = u[d];
+                });
+                a.rho[n] = rho;
+            }
+            return;
+        }
+    }
 
     // ---- moments, in the reference's summation order
     C f[Q];
@@ -67,10 +94,9 @@ __device__ __forceinline__ void node_update(const StepArgs<typename St::C> &a, i
     // FP32 compute regroups the arithmetic (one reciprocal of rho, paired
     // equilibria, the closed-form Smagorinsky norm); FP64 keeps the
     // reference's operation order bit for bit.
-    constexpr bool FAST = std::is_same<C, float>::value;
     C u[3];
     if constexpr (FAST) {
-        const C ir = C(1) / rho;
+        const C ir = recip(rho);
         sfor<0, D>([&](auto dd) { u[decltype(dd)::value] = j[decltype(dd)::value] * ir; });
     } else {
         sfor<0, D>([&](auto dd) {
@@ -80,7 +106,6 @@ __device__ __forceinline__ void node_update(const StepArgs<typename St::C> &a, i
     }
 
     // ---- node class
-    const unsigned m = a.mask[n];
     const uint16_t desc = m ? a.cls[m] : (uint16_t)0;
 
     if (desc & CLS_SOLID) {
@@ -145,11 +170,11 @@ __device__ __forceinline__ void node_update(const StepArgs<typename St::C> &a, i
             else equilibrium<L>(rho, u, feq);
             if constexpr (SMAG) {
                 if (a.c_smag > C(0)) {
-                    if constexpr (FAST) tau = smagorinsky_tau_fast<L>(tau, C(1) / rho, a.c_smag, f, feq);
+                    if constexpr (FAST) tau = smagorinsky_tau_fast<L>(tau, recip(rho), a.c_smag, f, feq);
                     else tau = smagorinsky_tau<L>(tau, rho, a.c_smag, f, feq);
                 }
             }
-            const C omega = C(1.0) / tau;
+            const C omega = FAST ? recip(tau) : C(1.0) / tau;
             if (a.porous) {
                 const C ob = C(1) - B;
                 C t[Q];
@@ -168,7 +193,9 @@ __device__ __forceinline__ void node_update(const StepArgs<typename St::C> &a, i
             } else {
                 sfor<0, Q>([&](auto ii) {
                     constexpr int i = decltype(ii)::value;
-                    C r = f[i] - omega * (f[i] - feq[i]);
+                    C r;
+                    if constexpr (FAST) r = f[i] + omega * (feq[i] - f[i]);
+                    else r = f[i] - omega * (f[i] - feq[i]);
                     if constexpr (GUO) r = r + ft[i];
                     f[i] = r;
                 });
@@ -220,6 +247,38 @@ __device__ __forceinline__ void node_update(const StepArgs<typename St::C> &a, i
     }
 }
 
+// gather (solver.cu:77-94), update, scatter to the same slots (solver.cu:119-123)
+template <class L, class St, int FAM, bool GUO, bool SMAG, bool MFIX, bool WIDE>
+__device__ __forceinline__ void step_node(const StepArgs<typename St::C> &a, int n, const int (&dd)[3][3])
+{
+    using C = typename St::C;
+    using S = typename St::S;
+    constexpr int Q = L::Q;
+    S *const fb = (S *)a.f;
+    C v[Q];
+    sfor<0, Q>([&](auto ii) {
+        constexpr int i = decltype(ii)::value;
+        const int k = n + dd[0][L::ex[i] + 1] + dd[1][L::ey[i] + 1] + dd[2][L::ez[i] + 1];
+        if constexpr (WIDE) {
+            v[i] = St::load(fb[(size_t)i * a.npad + k]);
+        } else {
+            const char *base = (const char *)(fb + (size_t)i * a.npad);
+            v[i] = St::load(*(const S *)(base + (unsigned)k * (unsigned)sizeof(S)));
+        }
+    });
+    node_update<L, St, FAM, GUO, SMAG, MFIX>(a, n, v);
+    sfor<0, Q>([&](auto ii) {
+        constexpr int i = decltype(ii)::value;
+        const int k = n + dd[0][L::ex[i] + 1] + dd[1][L::ey[i] + 1] + dd[2][L::ez[i] + 1];
+        if constexpr (WIDE) {
+            fb[(size_t)i * a.npad + k] = St::store(v[i]);
+        } else {
+            char *base = (char *)(fb + (size_t)i * a.npad);
+            *(S *)(base + (unsigned)k * (unsigned)sizeof(S)) = St::store(v[i]);
+        }
+    });
+}
+
 __device__ __forceinline__ int fast_div(int n, unsigned m, int l)
 {
     return (int)((__umulhi((unsigned)n, m) + (unsigned)n) >> l);
@@ -229,13 +288,9 @@ __device__ __forceinline__ int fast_div(int n, unsigned m, int l)
 // direction, consecutive nodes read consecutive slots of the source row
 // (rotated by t*e_x), so every wave's accesses are contiguous apart from
 // at most one wrap point.
-template <class L, class St, int FAM, bool GUO, bool SMAG, bool MFIX>
+template <class L, class St, int FAM, bool GUO, bool SMAG, bool MFIX, bool WIDE>
 __global__ __launch_bounds__(kBlock) void step_kernel(const StepArgs<typename St::C> a)
 {
-    using C = typename St::C;
-    using S = typename St::S;
-    constexpr int Q = L::Q;
-
     int b = (int)blockIdx.x;
     if (a.remap) {
         // blocks are dispatched round-robin over the 8 XCDs; give each XCD a
@@ -246,46 +301,32 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const StepArgs<typename St
     }
     const int n = b * kBlock + (int)threadIdx.x;
     if (n >= a.nnodes) return;
-    const int Lx = a.L[0], Ly = a.L[1], Lz = a.L[2];
+    const int Lx = a.L[0], Ly = a.L[1];
     const int row = fast_div(n, a.mx, a.lx);
     const int x = n - row * Lx;
     const int z = fast_div(row, a.my, a.ly);
     const int y = row - z * Ly;
 
-    // source coordinates for e in {-1, 0, 1} (index e+1), solver.cu:77-92
-    int xx[3], yy[3], zz[3];
-    xx[1] = x;
-    xx[2] = x - a.s[0];
-    if (xx[2] < 0) xx[2] += Lx;
-    xx[0] = x + a.s[0];
-    if (xx[0] >= Lx) xx[0] -= Lx;
-    yy[1] = y;
-    yy[2] = y - a.s[1];
-    if (yy[2] < 0) yy[2] += Ly;
-    yy[0] = y + a.s[1];
-    if (yy[0] >= Ly) yy[0] -= Ly;
-    zz[1] = z;
-    zz[2] = z - a.s[2];
-    if (zz[2] < 0) zz[2] += Lz;
-    zz[0] = z + a.s[2];
-    if (zz[0] >= Lz) zz[0] -= Lz;
-
-    S *const fb = (S *)a.f;
-    C v[Q];
-    sfor<0, Q>([&](auto ii) {
-        constexpr int i = decltype(ii)::value;
-        const int id = (zz[L::ez[i] + 1] * Ly + yy[L::ey[i] + 1]) * Lx + xx[L::ex[i] + 1];
-        v[i] = St::load(fb[(size_t)i * a.npad + id]);
+    // slot of direction i = n + DX[e_x] + DY[e_y] + DZ[e_z] (index e+1): the
+    // reference's ((c - t e) mod L) per axis (solver.cu:77-92), as offsets
+    // from the node's own slot, wrap included
+    const int c3[3] = {x, y, z};
+    int dd[3][3];
+    sfor<0, L::D>([&](auto aa) {
+        constexpr int ax = decltype(aa)::value;
+        const int c = c3[ax];
+        dd[ax][1] = 0;
+        dd[ax][2] = (c >= a.s[ax]) ? a.dp[ax] : a.dpw[ax];
+        dd[ax][0] = (c + a.s[ax] < a.L[ax]) ? a.dm[ax] : a.dmw[ax];
     });
+    if constexpr (L::D == 2) {
+        dd[2][0] = dd[2][1] = dd[2][2] = 0;
+    }
 
-    node_update<L, St, FAM, GUO, SMAG, MFIX>(a, n, v);
-
-    // back to the same slots, solver.cu:119-123
-    sfor<0, Q>([&](auto ii) {
-        constexpr int i = decltype(ii)::value;
-        const int id = (zz[L::ez[i] + 1] * Ly + yy[L::ey[i] + 1]) * Lx + xx[L::ex[i] + 1];
-        fb[(size_t)i * a.npad + id] = St::store(v[i]);
-    });
+    // WIDE = false (direction arrays below 4 GiB): a uniform base per
+    // direction plus a 32-bit byte offset per lane, no 64-bit address
+    // arithmetic per access
+    step_node<L, St, FAM, GUO, SMAG, MFIX, WIDE>(a, n, dd);
 }
 
 // Gathers the post-stream populations of the neighbours of the sf nodes

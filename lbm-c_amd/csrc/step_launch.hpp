@@ -8,9 +8,12 @@
 namespace lbmc {
 
 template <class L, class St, int FAM, bool GUO, bool SMAG, bool MFIX>
-static hipError_t launch_one(const StepArgs<typename St::C> &a, int grid, hipStream_t s)
+static hipError_t launch_one(const StepArgs<typename St::C> &a, int grid, hipStream_t s, bool wide)
 {
-    hipLaunchKernelGGL((step_kernel<L, St, FAM, GUO, SMAG, MFIX>), dim3(grid), dim3(kBlock), 0, s, a);
+    if (wide)
+        hipLaunchKernelGGL((step_kernel<L, St, FAM, GUO, SMAG, MFIX, true>), dim3(grid), dim3(kBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL((step_kernel<L, St, FAM, GUO, SMAG, MFIX, false>), dim3(grid), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }
 
@@ -20,20 +23,20 @@ static hipError_t launch_lattice(const KernelKey &k, const StepArgs<typename St:
 {
     if (k.family == 0) {
         if (k.guo) {
-            return k.smag ? launch_one<L, St, 0, true, true, false>(a, grid, s)
-                          : launch_one<L, St, 0, true, false, false>(a, grid, s);
+            return k.smag ? launch_one<L, St, 0, true, true, false>(a, grid, s, k.wide != 0)
+                          : launch_one<L, St, 0, true, false, false>(a, grid, s, k.wide != 0);
         }
-        return k.smag ? launch_one<L, St, 0, false, true, false>(a, grid, s)
-                      : launch_one<L, St, 0, false, false, false>(a, grid, s);
+        return k.smag ? launch_one<L, St, 0, false, true, false>(a, grid, s, k.wide != 0)
+                      : launch_one<L, St, 0, false, false, false>(a, grid, s, k.wide != 0);
     }
     if constexpr (L::D == 3) {
         if (k.mfix) {
-            return k.guo ? launch_one<L, St, 1, true, false, true>(a, grid, s)
-                         : launch_one<L, St, 1, false, false, true>(a, grid, s);
+            return k.guo ? launch_one<L, St, 1, true, false, true>(a, grid, s, k.wide != 0)
+                         : launch_one<L, St, 1, false, false, true>(a, grid, s, k.wide != 0);
         }
     }
-    return k.guo ? launch_one<L, St, 1, true, false, false>(a, grid, s)
-                 : launch_one<L, St, 1, false, false, false>(a, grid, s);
+    return k.guo ? launch_one<L, St, 1, true, false, false>(a, grid, s, k.wide != 0)
+                 : launch_one<L, St, 1, false, false, false>(a, grid, s, k.wide != 0);
 }
 
 template <class St>

@@ -70,6 +70,22 @@ def test_fp32_tolerance(case):
     assert du <= K
 
 
+@pytest.mark.parametrize("case", [cases.turbulence_d3q15(L=10, steps=12), cases.pressure_d2q9(steps=50)],
+                         ids=["turb", "pressure"])
+def test_wide_offset_kernels(case, monkeypatch):
+    """The 64-bit-offset kernels (used when a direction array spans 4 GiB or
+    more) forced on a small lattice: FP64 bit-exact, FP32 equal to the
+    32-bit-offset kernels bit for bit."""
+    o, g32 = run_pair(case, lbmc.FP32)
+    monkeypatch.setenv("LBMC_FORCE_WIDE", "1")
+    g64 = cases.build(lbmc.Lattice, case, storage=lbmc.FP64)
+    g64.run(0, case["steps"], 1)
+    assert np.array_equal(o.get_f(), g64.get_f())
+    gw = cases.build(lbmc.Lattice, case, storage=lbmc.FP32)
+    gw.run(0, case["steps"], 1)
+    assert np.array_equal(g32.get_f(), gw.get_f())
+
+
 def test_node_types_and_links_bitexact():
     for case in CASES:
         print(f"[mask] {case['name']}", flush=True)
