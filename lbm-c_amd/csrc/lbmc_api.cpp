@@ -58,6 +58,14 @@ struct lbmc_ctx {
     int grid = 1, remap = 0;
     FastDiv dx{}, dy{};
 
+    // captured step sequence: `period` calls starting at t = phase (mod
+    // period) with the given store pattern; replayed while valid
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    long long graph_period = 0, graph_phase = -1;
+    int graph_store = 0;
+    int use_graph = 1;
+
     std::string err;
 };
 
@@ -220,6 +228,73 @@ void choose_shape(lbmc_ctx *c)
         if (atoi(env) != 0 && c->grid % 8 == 0) c->remap = 1;
 }
 
+void drop_graph(lbmc_ctx *c)
+{
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+    c->graph_phase = -1;
+}
+
+long long lcm_ll(long long a, long long b)
+{
+    long long x = a, y = b;
+    while (y) {
+        const long long r = x % y;
+        x = y;
+        y = r;
+    }
+    return a / x * b;
+}
+
+// The kernel arguments of call t depend on t only through t mod L_d and,
+// for the macro store, (t + 1) mod store_every.  A run of P = lcm of those
+// calls therefore repeats exactly: capture it once into a graph and replay.
+lbmc_status run_steps(lbmc_ctx *c, long long t0, int nsteps, int store_every)
+{
+    long long period = lcm_ll(lcm_ll(c->L[0], c->L[1]), c->L[2]);
+    if (store_every > 0) period = lcm_ll(period, store_every);
+    const bool graphable = c->use_graph && period <= 4096 && nsteps >= 2 * period;
+    long long t = t0;
+    long long left = nsteps;
+    if (graphable) {
+        const long long phase = t0 % period;
+        const int sflag = store_every > 0 ? store_every : 0;
+        if (!c->graph_exec || c->graph_period != period || c->graph_phase != phase || c->graph_store != sflag) {
+            drop_graph(c);
+            HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+            lbmc_status st = LBMC_OK;
+            for (long long k = 0; k < period && st == LBMC_OK; k++) {
+                const long long tt = t0 + k;
+                st = launch_step(c, tt, store_every > 0 && ((tt + 1) % store_every) == 0);
+            }
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(c->stream, &g);
+            if (st != LBMC_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return st;
+            }
+            if (e != hipSuccess) return hip_fail(c, e, "graph capture");
+            c->graph = g;
+            HIP_TRY(c, hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0), "graph instantiate");
+            c->graph_period = period;
+            c->graph_phase = phase;
+            c->graph_store = sflag;
+        }
+        while (left >= period) {
+            HIP_TRY(c, hipGraphLaunch(c->graph_exec, c->stream), "graph launch");
+            t += period;
+            left -= period;
+        }
+    }
+    for (; left > 0; left--, t++) {
+        const int store = store_every > 0 && ((t + 1) % store_every) == 0;
+        if (lbmc_status st = launch_step(c, t, store)) return st;
+    }
+    return LBMC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -273,6 +348,7 @@ lbmc_status lbmc_create(const lbmc_desc *desc, lbmc_ctx **out)
     if (const char *env = getenv("LBMC_FORCE_WIDE"))
         if (atoi(env) != 0) c->key.wide = 1;
     choose_shape(c);
+    if (const char *env = getenv("LBMC_GRAPH")) c->use_graph = atoi(env) != 0;
 
     auto bail = [&](lbmc_status st) {
         g_last_error = c->err;
@@ -318,6 +394,7 @@ lbmc_status lbmc_destroy(lbmc_ctx *c)
 {
     if (!c) return LBMC_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graph(c);
     free_dev(c->f);
     void *p;
     p = c->mask; free_dev(p); c->mask = nullptr;
@@ -475,6 +552,7 @@ lbmc_status lbmc_set_fields(lbmc_ctx *c, const double *geometry, const int *macr
         HIP_TRY(c, hipMemcpy(c->sfidx, idx.data(), sizeof(int) * (size_t)N, hipMemcpyHostToDevice), "sf upload");
     }
     HIP_TRY(c, hipDeviceSynchronize(), "set_fields sync");
+    drop_graph(c);
     c->fields_set = true;
     return LBMC_OK;
 }
@@ -551,12 +629,7 @@ lbmc_status lbmc_run(lbmc_ctx *c, long long t0, int nsteps, int store_every)
     if (!c->fields_set) return fail(c, LBMC_ERR_STATE, "lbmc_set_fields must come before stepping");
     if (t0 < 0 || nsteps < 0) return fail(c, LBMC_ERR_INVALID, "negative time index or step count");
     HIP_TRY(c, hipSetDevice(c->desc.device), "hipSetDevice");
-    for (int k = 0; k < nsteps; k++) {
-        const long long t = t0 + k;
-        const int store = store_every > 0 && ((t + 1) % store_every) == 0;
-        if (lbmc_status st = launch_step(c, t, store)) return st;
-    }
-    return LBMC_OK;
+    return run_steps(c, t0, nsteps, store_every);
 }
 
 lbmc_status lbmc_time_steps(lbmc_ctx *c, long long t0, int nsteps, int store_every, float *ms)

@@ -1,0 +1,86 @@
+"""GPU checks at the benchmark configurations' real sizes.
+
+C1 (D2Q9 BGK + Guo body-force channel, 256x64, tau 0.8, g 1e-6, 10,000
+steps): the flow is uniform in x, so the FP64 oracle on a 4-column channel
+computes every column of the 256-column one.  Bars:
+  * FP64 storage: every node bit-identical to the oracle's column;
+  * FP32 storage: max |u - u_oracle| <= 2e-3 max|u|;
+  * both against the analytic start-up series
+      u(eta,t) = sum_{n odd} 4 g H^2/(nu pi^3 n^3) sin(n pi eta/H) (1 - exp(-n^2 pi^2 nu t/H^2))
+    with walls halfway between the solid and fluid rows (H = 62,
+    eta = y - 1/2): deviation at most the oracle's own deviation plus
+    2e-3 max|u|.
+Graph replay: lbmc_run replays a captured graph of lcm(Lx, Ly, Lz) calls;
+it must give the same bits as launching every call.
+"""
+import numpy as np
+import pytest
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+lbmc = pytest.importorskip("lbmc")
+import pyoracle  # noqa: E402
+
+
+def analytic_channel(Ly, tau, g, T, nterms=400):
+    nu = (tau - 0.5) / 3.0
+    H = Ly - 2.0
+    eta = np.arange(Ly) - 0.5
+    u = np.zeros(Ly)
+    for n in range(1, 2 * nterms, 2):
+        u += (4 * g * H * H / (nu * np.pi ** 3 * n ** 3) * np.sin(n * np.pi * eta / H)
+              * (1 - np.exp(-n * n * np.pi ** 2 * nu * T / (H * H))))
+    return u
+
+
+def test_c1_poiseuille_channel():
+    Lx, Ly, tau, g, T = 256, 64, 0.8, 1e-6, 10000
+    ref = cases.build(pyoracle.Oracle, cases.channel_d2q9(Lx=4, Ly=Ly, tau=tau, g=g))
+    ref.run(0, T, T)
+    ro, uo = ref.get_macros()
+    fo = ref.get_f().reshape(9, Ly, 4)[:, :, 0]
+    uo = uo[0].reshape(Ly, 4)[:, 0]
+    ua = analytic_channel(Ly, tau, g, T)
+    fluid = slice(1, Ly - 1)
+    umax = np.max(np.abs(uo))
+    dev_o = np.max(np.abs(uo[fluid] - ua[fluid]))
+    case = cases.channel_d2q9(Lx=Lx, Ly=Ly, tau=tau, g=g)
+    for storage in (lbmc.FP64, lbmc.FP32):
+        lat = cases.build(lbmc.Lattice, case, storage=storage)
+        lat.run(0, T, T)
+        rho, u = lat.get_macros()
+        ux = u[0].reshape(Ly, Lx)
+        f = lat.get_f().reshape(9, Ly, Lx)
+        # uniform in x
+        assert np.max(np.abs(ux - ux[:, :1])) <= (0 if storage == lbmc.FP64 else 1e-9)
+        dev = np.max(np.abs(ux[fluid, 0] - ua[fluid]))
+        err = np.max(np.abs(ux[:, 0] - uo))
+        print(f"C1 {lbmc.STORAGE_NAMES[storage]}: max|u - oracle| {err:.3e}, "
+              f"max|u - analytic| {dev:.3e} (oracle {dev_o:.3e}), u_max {umax:.4e}", flush=True)
+        if storage == lbmc.FP64:
+            assert np.array_equal(f, np.repeat(fo[:, :, None], Lx, axis=2))
+            assert np.array_equal(ux, np.repeat(uo[:, None], Lx, axis=1))
+        else:
+            assert err <= 2e-3 * umax
+        assert dev <= dev_o + 2e-3 * umax
+        lat.close()
+
+
+@pytest.mark.parametrize("storage", [0, 1], ids=["fp64", "fp32"])
+def test_graph_replay_matches_direct(storage, monkeypatch):
+    case = cases.pressure_d2q9(Lx=24, Ly=12, collision=cases.MRT, steps=0)
+    period = 24  # lcm(24, 12); store every 6 keeps the period
+    n = 3 * period + 5
+    monkeypatch.setenv("LBMC_GRAPH", "0")
+    d = cases.build(lbmc.Lattice, case, storage=storage)
+    d.run(7, n, 6)
+    monkeypatch.setenv("LBMC_GRAPH", "1")
+    g = cases.build(lbmc.Lattice, case, storage=storage)
+    g.run(7, n, 6)
+    g.run(7 + n, n, 6)  # second call reuses the captured graph
+    d.run(7 + n, n, 6)
+    assert np.array_equal(d.get_f(), g.get_f())
+    assert np.array_equal(d.get_macros()[0], g.get_macros()[0])
+    assert np.array_equal(d.get_macros()[1], g.get_macros()[1])
