@@ -1,0 +1,116 @@
+"""FP64 reference of the learned relaxation-time closure's MLP.
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP training and inference
+kernels (tests/ and bench-side experiments); the product never imports it.
+
+Network: 7 -> 64 -> 64 -> 1, activation GELU(x) = x Phi(x) with Phi
+evaluated as the device does: x clamped to [-3.5, 3.5], an odd degree-9
+polynomial (fixed FP32 coefficients, least-squares fit of the normal CDF on
+[-3.5, 3.5]) added to 1/2 and clamped to [0, 1]; |error| <= 8.6e-4 in Phi,
+<= 2.9e-3 in GELU (tests/test_closure_math.py checks both against erf).
+
+Parameters (one dict of float64 arrays):
+    w1 (64, 8)  hidden x (7 normalised inputs + a constant-1 bias column)
+    w2 (64, 64), b2 (64,), w3 (64,), b3 ()
+
+`quant=True` rounds, as the device does, the inputs, the weights, both
+hidden activations and both hidden-layer error signals to bfloat16 (round
+to nearest even, through float32) before they enter a product; sums stay in
+float64.  The device accumulates in float32, so quantised results agree with
+the device to float32 summation error.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+GELU_R = 3.5
+GELU_C = np.array([0.3964735269546509, -0.061638616025447845, 0.007179733831435442,
+                   -0.0004616279620677233, 1.2130119102948811e-05])
+NPARAM = 64 * 8 + 64 * 64 + 64 + 64 + 1
+
+
+def bf16(a):
+    """Round to bfloat16 (nearest even) through float32, returned as float64."""
+    a32 = np.asarray(a, dtype=np.float32)
+    u = a32.view(np.uint32).astype(np.uint64)
+    r = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    out = (r.astype(np.uint32) << 16).view(np.float32)
+    out = np.where(np.isnan(a32), a32, out)
+    return out.astype(np.float64)
+
+
+def phi(x):
+    xc = np.clip(x, -GELU_R, GELU_R)
+    t = xc * xc
+    c = GELU_C
+    p = xc * (c[0] + t * (c[1] + t * (c[2] + t * (c[3] + t * c[4]))))
+    return np.clip(0.5 + p, 0.0, 1.0)
+
+
+def gelu(x):
+    return x * phi(x)
+
+
+def dgelu(x):
+    xc = np.clip(x, -GELU_R, GELU_R)
+    t = xc * xc
+    c = GELU_C
+    p = xc * (c[0] + t * (c[1] + t * (c[2] + t * (c[3] + t * c[4]))))
+    dp = c[0] + t * (3 * c[1] + t * (5 * c[2] + t * (7 * c[3] + t * 9 * c[4])))
+    inside = (np.abs(x) < GELU_R) & (0.5 + p > 0.0) & (0.5 + p < 1.0)
+    return phi(x) + x * np.where(inside, dp, 0.0)
+
+
+def init_params(seed=0):
+    rng = np.random.default_rng(seed)
+    w1 = rng.standard_normal((64, 8)) * np.sqrt(2.0 / 7.0)
+    w1[:, 7] = 0.0
+    return {"w1": w1, "w2": rng.standard_normal((64, 64)) * np.sqrt(2.0 / 64.0),
+            "b2": np.zeros(64), "w3": rng.standard_normal(64) * np.sqrt(1.0 / 64.0), "b3": 0.0}
+
+
+def flatten(p):
+    return np.concatenate([p["w1"].reshape(-1), p["w2"].reshape(-1), p["b2"], p["w3"],
+                           np.atleast_1d(p["b3"])])
+
+
+def unflatten(v):
+    v = np.asarray(v, dtype=np.float64)
+    o = 0
+    w1 = v[o:o + 512].reshape(64, 8); o += 512
+    w2 = v[o:o + 4096].reshape(64, 64); o += 4096
+    b2 = v[o:o + 64]; o += 64
+    w3 = v[o:o + 64]; o += 64
+    return {"w1": w1, "w2": w2, "b2": b2, "w3": w3, "b3": float(v[o])}
+
+
+def forward(p, x, quant=False):
+    """x: (S, 7) normalised features -> y (S,), cache."""
+    q = bf16 if quant else (lambda a: np.asarray(a, dtype=np.float64))
+    X = np.concatenate([x, np.ones((x.shape[0], 1))], axis=1)
+    Xq, W1, W2, w3 = q(X), q(p["w1"]), q(p["w2"]), q(p["w3"])
+    z1 = Xq @ W1.T
+    a1 = q(gelu(z1))
+    z2 = a1 @ W2.T + p["b2"]
+    a2 = q(gelu(z2))
+    y = a2 @ w3 + p["b3"]
+    return y, dict(Xq=Xq, W2=W2, w3=w3, z1=z1, a1=a1, z2=z2, a2=a2)
+
+
+def loss_and_grad(p, x, t, quant=False):
+    """Mean squared error over the S samples and its gradient (same dict form)."""
+    q = bf16 if quant else (lambda a: np.asarray(a, dtype=np.float64))
+    y, c = forward(p, x, quant)
+    S = x.shape[0]
+    loss = float(np.mean((y - t) ** 2))
+    dy = 2.0 * (y - t) / S
+    gw3 = c["a2"].T @ dy
+    gb3 = float(dy.sum())
+    dz2 = np.outer(dy, c["w3"]) * dgelu(c["z2"])
+    gb2 = dz2.sum(axis=0)
+    dz2q = q(dz2)
+    gw2 = dz2q.T @ c["a1"]
+    da1 = dz2q @ c["W2"]
+    dz1q = q(da1 * dgelu(c["z1"]))
+    gw1 = dz1q.T @ c["Xq"]
+    return loss, {"w1": gw1, "w2": gw2, "b2": gb2, "w3": gw3, "b3": gb3}
