@@ -79,6 +79,7 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-variants", action="store_true", help="skip the FP16S/BF16S lines")
+    ap.add_argument("--no-closure", action="store_true", help="skip the learned-closure step line")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     args = ap.parse_args()
 
@@ -146,6 +147,31 @@ def main():
             v.close()
             log(f"bench: {lbmc.STORAGE_NAMES[st]} {variants[lbmc.STORAGE_NAMES[st]]}")
 
+    closure_line = None
+    if not args.no_closure:
+        # same lattice and state, relaxation time from the closure network
+        # (He-initialised weights; the cost does not depend on their values
+        # beyond the share of nodes that fall back to Smagorinsky)
+        import closure
+
+        mu = np.zeros(7, dtype=np.float32)
+        mu[6] = 1.0
+        isd = np.full(7, 1e3, dtype=np.float32)
+        isd[6] = 1e2
+        model = lbmc.Mlp(closure.init_params(7), mu=mu, isd=isd, out_a=0.005, out_b=0.515)
+        c = make_lattice(lbmc, L, lbmc.FP32, fields)
+        c.set_closure(model)
+        c.time_steps(0, max(5, args.warmup // 5), 0)
+        csteps = max(20, args.steps // 5)
+        cms = c.time_steps(args.warmup, csteps, 0)
+        c.close()
+        model.close()
+        closure_line = {"MLUPS": round(N * csteps / (cms * 1e-3) / 1e6, 1),
+                        "ms_per_step": round(cms / csteps, 5),
+                        "vs_smagorinsky_step": round((cms / csteps) / ms_step, 3),
+                        "mfma_flop_per_node": 2 * (16 * 64 + 64 * 64 + 64 * 16)}
+        log(f"bench: closure {closure_line}")
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -176,6 +202,7 @@ def main():
                      "copy_kernel_GBs": round(copy_gbs, 1)},
         "cpu_baseline": cpu,
         "variants": variants,
+        "closure": closure_line,
         "finite": finite,
     }
     print(json.dumps(out), flush=True)

@@ -166,6 +166,64 @@ lbmc_status lbmc_info(const lbmc_ctx *ctx, long long *nodes, int *q, int *slot_b
  * roofline and the HBM-counter calibration.  Not part of the reference. */
 lbmc_status lbmc_copy_bandwidth(int device, size_t nbytes, int reps, float *ms);
 
+/* ---- Learned relaxation-time closure --------------------------------------
+ *
+ * Replaces the reference's Smagorinsky hook turbulent_viscosity
+ * (LBM-C/collision.cu:445-463, called by bgk_collision at :60-62) with a
+ * 7 -> 64 -> 64 -> 1 GELU perceptron evaluated on the matrix cores (bf16
+ * operands, FP32 accumulation).  Inputs per node: Pi_ab / rho for ab = xx,
+ * yy, zz, xy, yz, xz with Pi = sum_i e_i e_i (f_i - feq_i), then rho; each
+ * normalised as (x - mu) * isd.  Output y gives tau' = out_a * y + out_b;
+ * outside [tau_lo, tau_hi] the node falls back to the Smagorinsky tau
+ * (c_smag of the descriptor; tau when c_smag is 0).  The reference has no
+ * learned closure and no trainer: these entry points are additions.
+ *
+ * Parameters are one float array of LBMC_MLP_NPARAM values:
+ *   w1[64][8]  first layer, column 7 is its bias (the input "1")
+ *   w2[64][64] second layer, row = output unit; b2[64]
+ *   w3[64], b3 output layer
+ */
+#define LBMC_MLP_NPARAM 4737
+
+typedef struct lbmc_mlp lbmc_mlp;
+
+typedef struct lbmc_mlp_scaling {
+    float mu[7], isd[7];  /* input normalisation */
+    float out_a, out_b;   /* tau' = out_a * y + out_b */
+} lbmc_mlp_scaling;
+
+/* Trainer / model on one device: FP32 master weights, Adam state, and the
+ * bf16 operand image the kernels read.  sc may be null (identity). */
+lbmc_status lbmc_mlp_create(int device, const float *params, const lbmc_mlp_scaling *sc, lbmc_mlp **out);
+void lbmc_mlp_destroy(lbmc_mlp *m);
+const char *lbmc_mlp_last_error(const lbmc_mlp *m);
+lbmc_status lbmc_mlp_set_params(lbmc_mlp *m, const float *params); /* resets Adam */
+lbmc_status lbmc_mlp_get_params(lbmc_mlp *m, float *params);
+lbmc_status lbmc_mlp_set_scaling(lbmc_mlp *m, const lbmc_mlp_scaling *sc);
+
+/* Network outputs y[S] for raw features feat[S][7] (host arrays), by the
+ * same device routine the closure step uses. */
+lbmc_status lbmc_mlp_forward(lbmc_mlp *m, const float *feat, long long S, float *y);
+
+/* Mean squared error of y against target[S] and its gradient (one batch,
+ * S <= 2^24), by the training kernels; grad may be null. */
+lbmc_status lbmc_mlp_loss_grad(lbmc_mlp *m, const float *feat, const float *target, long long S, float *grad,
+                               double *loss);
+
+/* Training data kept on the device (S < 2^31 samples). */
+lbmc_status lbmc_mlp_set_data(lbmc_mlp *m, const float *feat, const float *target, long long S);
+
+/* Adam (beta1 0.9, beta2 0.999, eps 1e-8) over shuffled minibatches of the
+ * resident data; epoch_loss[e] (may be null) is the mean squared error seen
+ * during epoch e.  The shuffle is seeded. */
+lbmc_status lbmc_mlp_train(lbmc_mlp *m, int epochs, int batch, float lr, unsigned long long seed,
+                           double *epoch_loss);
+
+/* Turn the closure on for a lattice (BGK or NTPOR collision, FP32 / FP16S /
+ * BF16S storage, same device as the model), with the model's current
+ * weights, or off with model = null. */
+lbmc_status lbmc_set_closure(lbmc_ctx *ctx, const lbmc_mlp *model, double tau_lo, double tau_hi);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,84 @@
+// Fused step with the learned relaxation-time closure (mlp.hpp) in place of
+// the Smagorinsky hook of bgk_collision (LBM-C/collision.cu:60-62,445-463).
+//
+// Same gather / update / scatter as step_kernel, with three differences:
+//  * the network's weights (16.5 KB of operands) are staged once per block
+//    into LDS, so blocks loop over 256-node chunks (grid-stride) instead of
+//    one block per chunk;
+//  * every node of a wave evaluates the network together, in uniform control
+//    flow (the matrix-core operands span the wave); lanes past the last node
+//    evaluate a copy of the last node and write nothing;
+//  * plain fluid nodes relax with tau' = out_a * y + out_b, or with the
+//    Smagorinsky tau where tau' leaves [lo, hi] (tau when c_smag = 0);
+//    boundary and porous nodes take the reference-order update with the
+//    Smagorinsky hook.
+#pragma once
+
+#include "mlp.hpp"
+#include "step_kernel.hpp"
+
+namespace lbmc {
+
+template <class L, class St, bool GUO>
+__device__ __forceinline__ void closure_update(const StepArgs<float> &a, int n, bool live, float (&v)[L::Q],
+                                               const MlpInfer &w, mlp_v8b *xw)
+{
+    constexpr int D = L::D;
+    const unsigned m = a.mask[n];
+    float F[3] = {0.f, 0.f, 0.f};
+    if constexpr (GUO) {
+        sfor<0, D>([&](auto dd) {
+            constexpr int d = decltype(dd)::value;
+            F[d] = a.force[0] ? a.force[d][n] : a.F[d];
+        });
+    }
+    FastState<L> st;
+    fast_prepare<L, St::shifted, GUO>(v, F, st);
+    float x[7];
+    fast_closure_features<L, St::shifted>(st, x);
+    const float y = mlp_wave_infer(w, xw, mlp_input(w.sc, x));
+    if (!live) return;
+    if (m == 0) {
+        float t = w.sc.out_a * y + w.sc.out_b;
+        if (!(t >= w.sc.lo && t <= w.sc.hi))
+            t = a.c_smag > 0.f ? fast_smag_tau<L, St::shifted>(st, a.tau, a.c_smag) : a.tau;
+        fast_relax<L, St::shifted, GUO>(v, st, t, a.tau, F);
+        if (a.store) {
+            sfor<0, D>([&](auto dd) {
+                constexpr int d = decltype(dd)::value;
+                a.u[d][n] = st.u[d];
+            });
+            a.rho[n] = st.rho;
+        }
+    } else {
+        node_update<L, St, 0, GUO, true, false>(a, n, v);
+    }
+}
+
+template <class L, class St, bool GUO, bool WIDE>
+__global__ __launch_bounds__(kBlock, 4) void closure_step_kernel(const StepArgs<float> a)
+{
+    __shared__ MlpInfer w;
+    __shared__ mlp_v8b xb[kBlock];
+    {
+        const uint4 *src = (const uint4 *)a.mlp;
+        uint4 *dst = (uint4 *)&w;
+        for (int k = (int)threadIdx.x; k < (int)(sizeof(MlpInfer) / 16); k += kBlock) dst[k] = src[k];
+    }
+    __syncthreads();
+    mlp_v8b *xw = xb + (threadIdx.x & ~63u);
+    const int nchunks = (a.nnodes + kBlock - 1) / kBlock;
+    for (int b = (int)blockIdx.x; b < nchunks; b += (int)gridDim.x) {
+        const int n0 = b * kBlock + (int)threadIdx.x;
+        const bool live = n0 < a.nnodes;
+        const int n = live ? n0 : a.nnodes - 1;
+        int dd[3][3];
+        slot_offsets<L>(a, n, dd);
+        float v[L::Q];
+        gather_node<L, St, WIDE>(a, n, dd, v);
+        closure_update<L, St, GUO>(a, n, live, v, w, xw);
+        if (live) scatter_node<L, St, WIDE>(a, n, dd, v);
+    }
+}
+
+}  // namespace lbmc

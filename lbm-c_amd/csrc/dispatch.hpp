@@ -7,6 +7,7 @@
 
 #include <cstdint>
 
+#include "../../include/lbmc.h"
 #include "step_args.hpp"
 
 namespace lbmc {
@@ -45,5 +46,10 @@ hipError_t launch_energy(int dim, int cdouble, const void *rho, const void *ux, 
 hipError_t launch_links(int lattice, const uint8_t *mask, const uint16_t *cls, int Lx, int Ly, int Lz,
                         uint16_t *links, hipStream_t s);
 hipError_t launch_copy(const void *src, void *dst, size_t nbytes, hipStream_t s);
+
+// learned closure, mlp.hip: operand image of a model's current weights with
+// the fallback band, into device memory dst on stream s
+size_t mlp_image_bytes();
+lbmc_status mlp_build_image(const lbmc_mlp *m, void *dst, int device, double lo, double hi, hipStream_t s);
 
 }  // namespace lbmc

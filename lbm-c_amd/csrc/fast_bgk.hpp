@@ -81,43 +81,91 @@ struct FastMoments<D2Q9> {
     }
 };
 
-// Fluid-node BGK update on stored values v (f, or g = f - w when SH).
-// F is the node's force (Guo); returns the stored macros in rho / u.
-template <class L, bool SH, bool GUO, bool SMAG>
-__device__ __forceinline__ void fluid_bgk_fast(float *v, float tau, float c_smag, const float *F,
-                                               float &rho_out, float *u_out)
+// Moments and velocity of a fluid node from its stored values v (f, or
+// g = f - w when SH); u carries the Guo half-force shift (collision.cu:74-78).
+template <class L>
+struct FastState {
+    float rho, ir;
+    float u[3];
+    float P[6];  // second moment of the stored values: xx, yy, zz, xy, yz, xz
+};
+
+template <class L, bool SH, bool GUO>
+__device__ __forceinline__ void fast_prepare(const float *v, const float *F, FastState<L> &st)
+{
+    constexpr int D = L::D;
+    constexpr float sh = SH ? 1.f : 0.f;
+    float s, j[3];
+    FastMoments<L>::run(v, s, j, st.P);
+    st.rho = sh + s;
+    st.ir = recip(st.rho);
+    st.u[0] = j[0] * st.ir;
+    st.u[1] = j[1] * st.ir;
+    st.u[2] = D > 2 ? j[2] * st.ir : 0.f;
+    if constexpr (GUO) {
+        const float h = 0.5f * st.ir;
+        sfor<0, D>([&](auto dd) {
+            constexpr int d = decltype(dd)::value;
+            st.u[d] = st.u[d] + h * F[d];
+        });
+    }
+}
+
+// Smagorinsky relaxation time (collision.cu:445-463):
+// Pi_ab = P_ab - rho u_a u_b - delta_ab (rho - sh)/3, norm sum_ab sqrt(2 Pi_ab^2)
+template <class L, bool SH>
+__device__ __forceinline__ float fast_smag_tau(const FastState<L> &st, float tau, float c_smag)
+{
+    constexpr float sh = SH ? 1.f : 0.f;
+    const float rho = st.rho;
+    const float *u = st.u;
+    const float *P = st.P;
+    const float dg = (rho - sh) * (1.f / 3.f);
+    float sd = fabsf(P[0] - rho * u[0] * u[0] - dg) + fabsf(P[1] - rho * u[1] * u[1] - dg);
+    float so = fabsf(P[3] - rho * u[0] * u[1]);
+    if constexpr (L::D > 2) {
+        sd += fabsf(P[2] - rho * u[2] * u[2] - dg);
+        so += fabsf(P[4] - rho * u[1] * u[2]) + fabsf(P[5] - rho * u[0] * u[2]);
+    }
+    const float A = 36.f * (c_smag * c_smag) * (sd + 2.f * so) * st.ir;
+    return 0.5f * (tau + sqrtf(tau * tau + A));
+}
+
+// Non-equilibrium momentum flux per unit density, Pi_ab / rho (xx, yy, zz,
+// xy, yz, xz; the z components are zero on D2Q9), and rho: the inputs of
+// the learned relaxation-time closure.
+template <class L, bool SH>
+__device__ __forceinline__ void fast_closure_features(const FastState<L> &st, float (&x)[7])
+{
+    constexpr float sh = SH ? 1.f : 0.f;
+    const float rho = st.rho;
+    const float *u = st.u;
+    const float *P = st.P;
+    const float dg = (rho - sh) * (1.f / 3.f);
+    x[0] = (P[0] - rho * u[0] * u[0] - dg) * st.ir;
+    x[1] = (P[1] - rho * u[1] * u[1] - dg) * st.ir;
+    x[3] = (P[3] - rho * u[0] * u[1]) * st.ir;
+    if constexpr (L::D > 2) {
+        x[2] = (P[2] - rho * u[2] * u[2] - dg) * st.ir;
+        x[4] = (P[4] - rho * u[1] * u[2]) * st.ir;
+        x[5] = (P[5] - rho * u[0] * u[2]) * st.ir;
+    } else {
+        x[2] = x[4] = x[5] = 0.f;
+    }
+    x[6] = rho;
+}
+
+// BGK relaxation with relaxation time t (and the Guo source with the base
+// tau, collision.cu:69-111) on the stored values.
+template <class L, bool SH, bool GUO>
+__device__ __forceinline__ void fast_relax(float *v, const FastState<L> &st, float t, float tau,
+                                           const float *F)
 {
     constexpr int D = L::D;
     constexpr int Q = L::Q;
     constexpr float sh = SH ? 1.f : 0.f;
-    float s, j[3], P[6];
-    FastMoments<L>::run(v, s, j, P);
-    const float rho = sh + s;
-    const float ir = recip(rho);
-    float u[3] = {j[0] * ir, j[1] * ir, D > 2 ? j[2] * ir : 0.f};
-    if constexpr (GUO) {
-        // u <- u + F/(2 rho), collision.cu:74-78
-        const float h = 0.5f * ir;
-        sfor<0, D>([&](auto dd) {
-            constexpr int d = decltype(dd)::value;
-            u[d] = u[d] + h * F[d];
-        });
-    }
-    float t = tau;
-    if constexpr (SMAG) {
-        if (c_smag > 0.f) {
-            // Pi_ab = P_ab - rho u_a u_b - delta_ab (rho - sh)/3
-            const float dg = (rho - sh) * (1.f / 3.f);
-            float sd = fabsf(P[0] - rho * u[0] * u[0] - dg) + fabsf(P[1] - rho * u[1] * u[1] - dg);
-            float so = fabsf(P[3] - rho * u[0] * u[1]);
-            if constexpr (D > 2) {
-                sd += fabsf(P[2] - rho * u[2] * u[2] - dg);
-                so += fabsf(P[4] - rho * u[1] * u[2]) + fabsf(P[5] - rho * u[0] * u[2]);
-            }
-            const float A = 36.f * (c_smag * c_smag) * (sd + 2.f * so) * ir;
-            t = 0.5f * (tau + sqrtf(tau * tau + A));
-        }
-    }
+    const float rho = st.rho;
+    const float *u = st.u;
     const float om = recip(t);
     const float usq = u_square<L>(u);
     const float base = 1.f - usq;
@@ -157,8 +205,23 @@ __device__ __forceinline__ void fluid_bgk_fast(float *v, float tau, float c_smag
             v[o] = ro;
         }
     });
-    rho_out = rho;
-    sfor<0, D>([&](auto dd) { u_out[decltype(dd)::value] = u[decltype(dd)::value]; });
+}
+
+// Fluid-node BGK update on stored values v (f, or g = f - w when SH).
+// F is the node's force (Guo); returns the stored macros in rho / u.
+template <class L, bool SH, bool GUO, bool SMAG>
+__device__ __forceinline__ void fluid_bgk_fast(float *v, float tau, float c_smag, const float *F,
+                                               float &rho_out, float *u_out)
+{
+    FastState<L> st;
+    fast_prepare<L, SH, GUO>(v, F, st);
+    float t = tau;
+    if constexpr (SMAG) {
+        if (c_smag > 0.f) t = fast_smag_tau<L, SH>(st, tau, c_smag);
+    }
+    fast_relax<L, SH, GUO>(v, st, t, tau, F);
+    rho_out = st.rho;
+    sfor<0, L::D>([&](auto dd) { u_out[decltype(dd)::value] = st.u[decltype(dd)::value]; });
 }
 
 }  // namespace lbmc

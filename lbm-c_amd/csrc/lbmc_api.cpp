@@ -54,8 +54,12 @@ struct lbmc_ctx {
     uint16_t cls_host[256] = {};
     KernelKey key{};
 
+    // learned closure: operand image (device), on when key.closure
+    void *mlp_img = nullptr;
+
     // launch shape
     int grid = 1, remap = 0;
+    int closure_grid = 1;  // blocks of the grid-stride closure kernel
     FastDiv dx{}, dy{};
 
     // captured step sequence: `period` calls starting at t = phase (mod
@@ -181,6 +185,7 @@ void fill_args(const lbmc_ctx *c, StepArgs<C> &a, long long t, int store)
     a.store = store ? 1 : 0;
     a.porous = c->porous ? 1 : 0;
     a.quirks = c->desc.quirks;
+    a.mlp = c->mlp_img;
 }
 
 lbmc_status launch_step(lbmc_ctx *c, long long t, int store)
@@ -205,10 +210,11 @@ lbmc_status launch_step(lbmc_ctx *c, long long t, int store)
             }
             if (e != hipSuccess) return hip_fail(c, e, "sf gather launch");
         }
+        const int grid = c->key.closure ? c->closure_grid : c->grid;
         switch (c->storage) {
-        case LBMC_FP32: e = launch_step_fp32(c->key, a, c->grid, c->stream); break;
-        case LBMC_FP16S: e = launch_step_fp16s(c->key, a, c->grid, c->stream); break;
-        default: e = launch_step_bf16s(c->key, a, c->grid, c->stream); break;
+        case LBMC_FP32: e = launch_step_fp32(c->key, a, grid, c->stream); break;
+        case LBMC_FP16S: e = launch_step_fp16s(c->key, a, grid, c->stream); break;
+        default: e = launch_step_bf16s(c->key, a, grid, c->stream); break;
         }
     }
     if (e != hipSuccess) return hip_fail(c, e, "step kernel launch");
@@ -226,6 +232,12 @@ void choose_shape(lbmc_ctx *c)
     c->remap = 0;
     if (const char *env = getenv("LBMC_XCD_REMAP"))
         if (atoi(env) != 0 && c->grid % 8 == 0) c->remap = 1;
+    // the closure kernel stages its weights once per block and loops over
+    // 256-node chunks: a few blocks per CU
+    int cap = 2048;
+    if (const char *env = getenv("LBMC_CLOSURE_GRID"))
+        if (atoi(env) > 0) cap = atoi(env);
+    c->closure_grid = std::min(c->grid, cap);
 }
 
 void drop_graph(lbmc_ctx *c)
@@ -410,6 +422,7 @@ lbmc_status lbmc_destroy(lbmc_ctx *c)
     free_dev(c->sfbuf);
     p = c->partials; free_dev(p);
     p = c->scalar; free_dev(p);
+    free_dev(c->mlp_img);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -739,6 +752,31 @@ lbmc_status lbmc_kinetic_energy(lbmc_ctx *c, double *energy)
             "energy launch");
     HIP_TRY(c, hipMemcpyAsync(energy, c->scalar, sizeof(double), hipMemcpyDeviceToHost, c->stream), "energy read");
     HIP_TRY(c, hipStreamSynchronize(c->stream), "energy sync");
+    return LBMC_OK;
+}
+
+lbmc_status lbmc_set_closure(lbmc_ctx *c, const lbmc_mlp *model, double tau_lo, double tau_hi)
+{
+    if (!c) return fail(nullptr, LBMC_ERR_INVALID, "lbmc_set_closure: null context");
+    if (!model) {
+        c->key.closure = 0;
+        drop_graph(c);
+        return LBMC_OK;
+    }
+    if (c->storage == LBMC_FP64)
+        return fail(c, LBMC_ERR_UNSUPPORTED, "lbmc_set_closure: the closure computes in FP32; FP64 storage has none");
+    if (c->key.family != 0)
+        return fail(c, LBMC_ERR_UNSUPPORTED, "lbmc_set_closure: the closure replaces the BGK Smagorinsky hook (BGK/NTPOR only)");
+    if (!(tau_lo > 0.5) || !(tau_hi >= tau_lo) || !std::isfinite(tau_hi))
+        return fail(c, LBMC_ERR_INVALID, "lbmc_set_closure: band [%g, %g] must satisfy 0.5 < lo <= hi", tau_lo, tau_hi);
+    HIP_TRY(c, hipSetDevice(c->desc.device), "hipSetDevice");
+    if (!c->mlp_img) HIP_TRY(c, hipMalloc(&c->mlp_img, mlp_image_bytes()), "closure image allocation");
+    HIP_TRY(c, hipStreamSynchronize(c->stream), "closure sync");
+    const lbmc_status st = mlp_build_image(model, c->mlp_img, c->desc.device, tau_lo, tau_hi, c->stream);
+    if (st != LBMC_OK) return fail(c, st, "lbmc_set_closure: model is on another device or the image build failed");
+    HIP_TRY(c, hipStreamSynchronize(c->stream), "closure image build");
+    c->key.closure = 1;
+    drop_graph(c);
     return LBMC_OK;
 }
 

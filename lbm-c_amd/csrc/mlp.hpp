@@ -1,0 +1,212 @@
+// Learned relaxation-time closure: a 7 -> 64 -> 64 -> 1 perceptron with
+// GELU activations, evaluated and trained on the matrix cores.  It takes the
+// place of the reference's Smagorinsky hook (LBM-C/collision.cu:445-463,
+// called from bgk_collision at :60-62): tau' = out_a * y + out_b, and the
+// Smagorinsky tau is used wherever tau' leaves the band [lo, hi].
+//
+// Arithmetic: bf16 operands, FP32 accumulation (MFMA 16x16x16 bf16 for the
+// first layer, 16x16x32 bf16 for the others); biases, activations and their
+// derivatives in FP32.  GELU(x) = x Phi(x) with Phi from an odd degree-9
+// polynomial on x clamped to [-3.5, 3.5] (least-squares fit of the normal
+// CDF, |error| <= 8.6e-4), then clamped to [0, 1].
+//
+// Wave mapping (64 lanes, lane l = 16 g + r): a wave evaluates 64 samples
+// as four tiles of 16.  Each layer is computed transposed, H = W X, with
+// weights as the A operand and samples as the columns of B, so the MFMA
+// result layout of one layer (lane holds rows 4g..4g+3 of column r) is the
+// B-operand layout of the next: no data moves between lanes inside the
+// network.  For the K = 32 products the hidden units of the two 16-row
+// tiles 2c, 2c+1 are taken in the order hid(c, g, j) below, which is the
+// order in which lane g of the previous result already holds them.  The
+// output layer uses w3 on all 16 rows, so every lane receives y of column
+// r; lane 16g + r keeps tile g's value, which is its own sample.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace lbmc {
+
+constexpr int MLP_NIN = 7;
+constexpr int MLP_H = 64;
+// flat parameters: w1[64][8] (column 7 is the first-layer bias), w2[64][64]
+// (row = output unit), b2[64], w3[64], b3
+constexpr int MLP_W1 = 0, MLP_W2 = 512, MLP_B2 = 4608, MLP_W3 = 4672, MLP_B3 = 4736;
+constexpr int MLP_NPARAM = 4737;
+// per-wave partial gradient record: the parameters, then the squared-error sum
+constexpr int MLP_PSTRIDE = 4740;
+
+typedef short mlp_v4s __attribute__((ext_vector_type(4)));
+typedef __bf16 mlp_v8b __attribute__((ext_vector_type(8)));
+typedef float mlp_v4f __attribute__((ext_vector_type(4)));
+
+// hidden unit held at K position 8g + j of the 32-wide chunk c
+__host__ __device__ constexpr int mlp_hid(int c, int g, int j) { return 16 * (2 * c + (j >> 2)) + 4 * g + (j & 3); }
+
+struct MlpScalars {
+    float b3, out_a, out_b, lo, hi;
+    float mu[7], isd[7];  // input normalisation x = (feature - mu) * isd
+    float pad[13];
+};
+
+// Operand-ordered weights for the forward pass: entry [..][l] is what lane l
+// feeds to the matrix cores, so one LDS read per operand and lane.
+struct MlpInfer {
+    mlp_v8b w2[4][2][64];  // A of layer 2, row tile h, chunk c: W2[16h + r][hid(c, g, j)]
+    mlp_v8b w3[2][64];     // A of layer 3, chunk c: w3[hid(c, g, j)] on every row
+    mlp_v4f b2[4][64];     // layer-2 accumulator start: b2[16h + 4g + j]
+    mlp_v4s w1[4][64];     // A of layer 1, row tile h: W1[16h + r][4g + j] (zero for g >= 2)
+    MlpScalars sc;
+};
+
+struct MlpImage {
+    MlpInfer inf;
+    // backward only
+    mlp_v8b w2t[4][2][64];  // A of W2^T, row tile h, chunk c: W2[hid(c, g, j)][16h + r]
+    mlp_v4f w3d[4][64];     // w3[16h + 4g + j] as rounded to bf16, in FP32
+};
+
+static_assert(sizeof(MlpInfer) % 16 == 0, "MlpInfer is copied in 16-byte units");
+static_assert(sizeof(MlpImage) % 16 == 0, "MlpImage is copied in 16-byte units");
+
+// GELU coefficients: Phi(x) - 1/2 = x (c1 + t (c3 + t (c5 + t (c7 + t c9)))), t = x^2
+constexpr float GELU_R = 3.5f;
+constexpr float GELU_C1 = 0.3964735269546509f;
+constexpr float GELU_C3 = -0.061638616025447845f;
+constexpr float GELU_C5 = 0.007179733831435442f;
+constexpr float GELU_C7 = -0.0004616279620677233f;
+constexpr float GELU_C9 = 1.2130119102948811e-05f;
+
+__device__ __forceinline__ float gelu_fwd(float x)
+{
+    const float xc = fminf(fmaxf(x, -GELU_R), GELU_R);
+    const float t = xc * xc;
+    float p = fmaf(t, GELU_C9, GELU_C7);
+    p = fmaf(t, p, GELU_C5);
+    p = fmaf(t, p, GELU_C3);
+    p = fmaf(t, p, GELU_C1);
+    const float phi = fminf(fmaxf(fmaf(xc, p, 0.5f), 0.f), 1.f);
+    return x * phi;
+}
+
+// d GELU / dx = Phi(x) + x Phi'(x), Phi' zero where x or Phi is clamped
+__device__ __forceinline__ float gelu_grad(float x)
+{
+    const float xc = fminf(fmaxf(x, -GELU_R), GELU_R);
+    const float t = xc * xc;
+    float p = fmaf(t, GELU_C9, GELU_C7);
+    p = fmaf(t, p, GELU_C5);
+    p = fmaf(t, p, GELU_C3);
+    p = fmaf(t, p, GELU_C1);
+    const float s = fmaf(xc, p, 0.5f);
+    float dp = fmaf(t, 9.f * GELU_C9, 7.f * GELU_C7);
+    dp = fmaf(t, dp, 5.f * GELU_C5);
+    dp = fmaf(t, dp, 3.f * GELU_C3);
+    dp = fmaf(t, dp, GELU_C1);
+    const bool inside = fabsf(x) < GELU_R && s > 0.f && s < 1.f;
+    return fminf(fmaxf(s, 0.f), 1.f) + (inside ? x * dp : 0.f);
+}
+
+// round to nearest even bf16 (finite inputs)
+__host__ __device__ inline unsigned short bf16_bits(float x)
+{
+    const unsigned u = __builtin_bit_cast(unsigned, x);
+    return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+__device__ __forceinline__ mlp_v4f mfma_k16(mlp_v4s a, mlp_v4s b, mlp_v4f c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ mlp_v4f mfma_k32(mlp_v8b a, mlp_v8b b, mlp_v4f c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// LDS written by one lane and read by another of the same wave
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// a sample's raw features -> normalised bf16 input row (constant-1 bias input last)
+__device__ __forceinline__ mlp_v8b mlp_input(const MlpScalars &s, const float (&f)[7])
+{
+    mlp_v8b r;
+#pragma unroll
+    for (int k = 0; k < 7; k++) r[k] = (__bf16)((f[k] - s.mu[k]) * s.isd[k]);
+    r[7] = (__bf16)1.f;
+    return r;
+}
+
+// Forward values of one 16-sample tile, in result layout (lane 16g + r:
+// hidden rows 4g..4g+3 of sample column r).
+struct MlpTile {
+    mlp_v4f z1[4], z2[4];  // pre-activations of hidden tiles h (kept for the backward pass)
+    mlp_v8b a1[2], a2[2];  // bf16 activations as B operands of the next layer, chunk c
+    float y;               // output of sample column r, without b3
+};
+
+// Forward pass of tile T.  xb holds the wave's 64 input rows.  Every lane of
+// the wave must take part (the matrix-core operands span the wave).
+template <bool KEEP>
+__device__ __forceinline__ void mlp_tile_forward(const MlpInfer &w, const mlp_v8b *xb, int T, int l,
+                                                 MlpTile &o)
+{
+    const int g = l >> 4, r = l & 15;
+    const mlp_v4f zero = {0.f, 0.f, 0.f, 0.f};
+    // The weight operands are read from LDS at their point of use: an opaque
+    // lane index keeps the compiler from hoisting all of them out of a loop
+    // over tiles (64 registers held for the whole loop).
+    int lw = l;
+    asm volatile("" : "+v"(lw));
+    // layer-1 B operand: inputs 4g..4g+3 of sample 16T + r (K rows 8..15 are zero)
+    const mlp_v4s *row = (const mlp_v4s *)&xb[16 * T + r];
+    mlp_v4s xop = row[g & 1];
+    if (g >= 2) xop = mlp_v4s{0, 0, 0, 0};
+#pragma unroll
+    for (int h = 0; h < 4; h++) {
+        const mlp_v4f z = mfma_k16(w.w1[h][lw], xop, zero);
+        if constexpr (KEEP) o.z1[h] = z;
+#pragma unroll
+        for (int j = 0; j < 4; j++) o.a1[h >> 1][(h & 1) * 4 + j] = (__bf16)gelu_fwd(z[j]);
+    }
+#pragma unroll
+    for (int h = 0; h < 4; h++) {
+        mlp_v4f acc = w.b2[h][lw];
+        acc = mfma_k32(w.w2[h][0][lw], o.a1[0], acc);
+        acc = mfma_k32(w.w2[h][1][lw], o.a1[1], acc);
+        if constexpr (KEEP) o.z2[h] = acc;
+#pragma unroll
+        for (int j = 0; j < 4; j++) o.a2[h >> 1][(h & 1) * 4 + j] = (__bf16)gelu_fwd(acc[j]);
+    }
+    mlp_v4f yv = mfma_k32(w.w3[0][lw], o.a2[0], zero);
+    yv = mfma_k32(w.w3[1][lw], o.a2[1], yv);
+    o.y = yv[0];
+}
+
+// Network output for the lane's own sample (input row xin).  xb: the wave's
+// 64-entry LDS staging area.  Called by all 64 lanes together.
+__device__ __forceinline__ float mlp_wave_infer(const MlpInfer &w, mlp_v8b *xb, mlp_v8b xin)
+{
+    const int l = (int)(threadIdx.x & 63);
+    xb[l] = xin;
+    wave_sync();
+    float y = 0.f;
+    // tiles one after another: unrolling them lets the scheduler interleave
+    // four tiles' operands and halves the occupancy
+#pragma unroll 1
+    for (int T = 0; T < 4; T++) {
+        MlpTile o;
+        mlp_tile_forward<false>(w, xb, T, l, o);
+        if ((l >> 4) == T) y = o.y;
+    }
+    wave_sync();
+    return y + w.sc.b3;
+}
+
+}  // namespace lbmc

@@ -30,6 +30,7 @@ struct KernelKey {
     int smag;     // Smagorinsky (BGK family only)
     int mfix;     // corrected D3Q15 M^-1 (MRT family only)
     int wide;     // a direction array spans 4 GiB or more: 64-bit offsets
+    int closure;  // learned relaxation-time closure (BGK family, FP32 compute)
 };
 
 enum : unsigned {
@@ -68,6 +69,7 @@ struct StepArgs {
     int store;               // store rho/u this call
     int porous;              // NTPOR/MRTPOR with porous nodes present
     unsigned quirks;
+    const void *mlp;         // closure weights (MlpImage) when the closure is on
 };
 
 }  // namespace lbmc

@@ -247,16 +247,15 @@ __device__ __forceinline__ void node_update(const StepArgs<typename St::C> &a, i
     }
 }
 
-// gather (solver.cu:77-94), update, scatter to the same slots (solver.cu:119-123)
-template <class L, class St, int FAM, bool GUO, bool SMAG, bool MFIX, bool WIDE>
-__device__ __forceinline__ void step_node(const StepArgs<typename St::C> &a, int n, const int (&dd)[3][3])
+// index-shift gather of node n's populations (solver.cu:77-94) from the
+// slots n + dd[0][e_x+1] + dd[1][e_y+1] + dd[2][e_z+1]
+template <class L, class St, bool WIDE>
+__device__ __forceinline__ void gather_node(const StepArgs<typename St::C> &a, int n, const int (&dd)[3][3],
+                                            typename St::C (&v)[L::Q])
 {
-    using C = typename St::C;
     using S = typename St::S;
-    constexpr int Q = L::Q;
-    S *const fb = (S *)a.f;
-    C v[Q];
-    sfor<0, Q>([&](auto ii) {
+    const S *const fb = (const S *)a.f;
+    sfor<0, L::Q>([&](auto ii) {
         constexpr int i = decltype(ii)::value;
         const int k = n + dd[0][L::ex[i] + 1] + dd[1][L::ey[i] + 1] + dd[2][L::ez[i] + 1];
         if constexpr (WIDE) {
@@ -266,8 +265,16 @@ __device__ __forceinline__ void step_node(const StepArgs<typename St::C> &a, int
             v[i] = St::load(*(const S *)(base + (unsigned)k * (unsigned)sizeof(S)));
         }
     });
-    node_update<L, St, FAM, GUO, SMAG, MFIX>(a, n, v);
-    sfor<0, Q>([&](auto ii) {
+}
+
+// write-back to the same slots (solver.cu:119-123)
+template <class L, class St, bool WIDE>
+__device__ __forceinline__ void scatter_node(const StepArgs<typename St::C> &a, int n, const int (&dd)[3][3],
+                                             const typename St::C (&v)[L::Q])
+{
+    using S = typename St::S;
+    S *const fb = (S *)a.f;
+    sfor<0, L::Q>([&](auto ii) {
         constexpr int i = decltype(ii)::value;
         const int k = n + dd[0][L::ex[i] + 1] + dd[1][L::ey[i] + 1] + dd[2][L::ez[i] + 1];
         if constexpr (WIDE) {
@@ -279,9 +286,44 @@ __device__ __forceinline__ void step_node(const StepArgs<typename St::C> &a, int
     });
 }
 
+// gather, update, scatter
+template <class L, class St, int FAM, bool GUO, bool SMAG, bool MFIX, bool WIDE>
+__device__ __forceinline__ void step_node(const StepArgs<typename St::C> &a, int n, const int (&dd)[3][3])
+{
+    typename St::C v[L::Q];
+    gather_node<L, St, WIDE>(a, n, dd, v);
+    node_update<L, St, FAM, GUO, SMAG, MFIX>(a, n, v);
+    scatter_node<L, St, WIDE>(a, n, dd, v);
+}
+
 __device__ __forceinline__ int fast_div(int n, unsigned m, int l)
 {
     return (int)((__umulhi((unsigned)n, m) + (unsigned)n) >> l);
+}
+
+// Slot offsets of node n for this call: the slot of direction i is
+// n + dd[0][e_x+1] + dd[1][e_y+1] + dd[2][e_z+1], the reference's
+// ((c - t e) mod L) per axis (solver.cu:77-92) relative to the node's own
+// slot, wrap included.
+template <class L, class C>
+__device__ __forceinline__ void slot_offsets(const StepArgs<C> &a, int n, int (&dd)[3][3])
+{
+    const int Lx = a.L[0], Ly = a.L[1];
+    const int row = fast_div(n, a.mx, a.lx);
+    const int x = n - row * Lx;
+    const int z = fast_div(row, a.my, a.ly);
+    const int y = row - z * Ly;
+    const int c3[3] = {x, y, z};
+    sfor<0, L::D>([&](auto aa) {
+        constexpr int ax = decltype(aa)::value;
+        const int c = c3[ax];
+        dd[ax][1] = 0;
+        dd[ax][2] = (c >= a.s[ax]) ? a.dp[ax] : a.dpw[ax];
+        dd[ax][0] = (c + a.s[ax] < a.L[ax]) ? a.dm[ax] : a.dmw[ax];
+    });
+    if constexpr (L::D == 2) {
+        dd[2][0] = dd[2][1] = dd[2][2] = 0;
+    }
 }
 
 // One thread per node, nodes in slot order (x fastest).  For a fixed
@@ -301,27 +343,8 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const StepArgs<typename St
     }
     const int n = b * kBlock + (int)threadIdx.x;
     if (n >= a.nnodes) return;
-    const int Lx = a.L[0], Ly = a.L[1];
-    const int row = fast_div(n, a.mx, a.lx);
-    const int x = n - row * Lx;
-    const int z = fast_div(row, a.my, a.ly);
-    const int y = row - z * Ly;
-
-    // slot of direction i = n + DX[e_x] + DY[e_y] + DZ[e_z] (index e+1): the
-    // reference's ((c - t e) mod L) per axis (solver.cu:77-92), as offsets
-    // from the node's own slot, wrap included
-    const int c3[3] = {x, y, z};
     int dd[3][3];
-    sfor<0, L::D>([&](auto aa) {
-        constexpr int ax = decltype(aa)::value;
-        const int c = c3[ax];
-        dd[ax][1] = 0;
-        dd[ax][2] = (c >= a.s[ax]) ? a.dp[ax] : a.dpw[ax];
-        dd[ax][0] = (c + a.s[ax] < a.L[ax]) ? a.dm[ax] : a.dmw[ax];
-    });
-    if constexpr (L::D == 2) {
-        dd[2][0] = dd[2][1] = dd[2][2] = 0;
-    }
+    slot_offsets<L>(a, n, dd);
 
     // WIDE = false (direction arrays below 4 GiB): a uniform base per
     // direction plus a 32-bit byte offset per lane, no 64-bit address

@@ -1,7 +1,11 @@
 // Variant selection for the fused step: one instantiation per lattice,
-// collision family, Guo forcing, Smagorinsky and D3Q15 M^-1 choice.
+// collision family, Guo forcing, Smagorinsky and D3Q15 M^-1 choice, plus the
+// learned-closure step (FP32-compute storages).
 #pragma once
 
+#include <type_traits>
+
+#include "closure_kernel.hpp"
 #include "dispatch.hpp"
 #include "step_kernel.hpp"
 
@@ -17,10 +21,26 @@ static hipError_t launch_one(const StepArgs<typename St::C> &a, int grid, hipStr
     return hipGetLastError();
 }
 
+template <class L, class St, bool GUO>
+static hipError_t launch_closure(const StepArgs<float> &a, int grid, hipStream_t s, bool wide)
+{
+    if (wide)
+        hipLaunchKernelGGL((closure_step_kernel<L, St, GUO, true>), dim3(grid), dim3(kBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL((closure_step_kernel<L, St, GUO, false>), dim3(grid), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
 template <class L, class St>
 static hipError_t launch_lattice(const KernelKey &k, const StepArgs<typename St::C> &a, int grid,
                                  hipStream_t s)
 {
+    if constexpr (std::is_same<typename St::C, float>::value) {
+        if (k.closure && k.family == 0) {
+            return k.guo ? launch_closure<L, St, true>(a, grid, s, k.wide != 0)
+                         : launch_closure<L, St, false>(a, grid, s, k.wide != 0);
+        }
+    }
     if (k.family == 0) {
         if (k.guo) {
             return k.smag ? launch_one<L, St, 0, true, true, false>(a, grid, s, k.wide != 0)

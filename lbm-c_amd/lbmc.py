@@ -36,7 +36,12 @@ EXPORTS = [
     "lbmc_get_links", "lbmc_init_static", "lbmc_init_equilibrium", "lbmc_step", "lbmc_run",
     "lbmc_time_steps", "lbmc_get_macros", "lbmc_get_node", "lbmc_get_f", "lbmc_set_f",
     "lbmc_kinetic_energy", "lbmc_sync", "lbmc_info", "lbmc_copy_bandwidth",
+    "lbmc_mlp_create", "lbmc_mlp_destroy", "lbmc_mlp_last_error", "lbmc_mlp_set_params",
+    "lbmc_mlp_get_params", "lbmc_mlp_set_scaling", "lbmc_mlp_forward", "lbmc_mlp_loss_grad",
+    "lbmc_mlp_set_data", "lbmc_mlp_train", "lbmc_set_closure",
 ]
+
+MLP_NPARAM = 4737
 
 
 class LbmcError(RuntimeError):
@@ -62,6 +67,11 @@ class Desc(ct.Structure):
     ]
 
 
+class MlpScaling(ct.Structure):
+    _fields_ = [("mu", ct.c_float * 7), ("isd", ct.c_float * 7), ("out_a", ct.c_float),
+                ("out_b", ct.c_float)]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
@@ -69,6 +79,7 @@ def _load():
     lib = ct.CDLL(LIB_PATH)
     P = ct.c_void_p
     dpp = ct.POINTER(ct.POINTER(ct.c_double))
+    fp = ct.POINTER(ct.c_float)
     sig = {
         "lbmc_create": (ct.c_int, [ct.POINTER(Desc), ct.POINTER(P)]),
         "lbmc_destroy": (ct.c_int, [P]),
@@ -95,6 +106,18 @@ def _load():
                                  ct.POINTER(ct.c_int)]),
         "lbmc_copy_bandwidth": (ct.c_int, [ct.c_int, ct.c_size_t, ct.c_int,
                                            ct.POINTER(ct.c_float)]),
+        "lbmc_mlp_create": (ct.c_int, [ct.c_int, fp, ct.POINTER(MlpScaling), ct.POINTER(P)]),
+        "lbmc_mlp_destroy": (None, [P]),
+        "lbmc_mlp_last_error": (ct.c_char_p, [P]),
+        "lbmc_mlp_set_params": (ct.c_int, [P, fp]),
+        "lbmc_mlp_get_params": (ct.c_int, [P, fp]),
+        "lbmc_mlp_set_scaling": (ct.c_int, [P, ct.POINTER(MlpScaling)]),
+        "lbmc_mlp_forward": (ct.c_int, [P, fp, ct.c_longlong, fp]),
+        "lbmc_mlp_loss_grad": (ct.c_int, [P, fp, fp, ct.c_longlong, fp, ct.POINTER(ct.c_double)]),
+        "lbmc_mlp_set_data": (ct.c_int, [P, fp, fp, ct.c_longlong]),
+        "lbmc_mlp_train": (ct.c_int, [P, ct.c_int, ct.c_int, ct.c_float, ct.c_ulonglong,
+                                      ct.POINTER(ct.c_double)]),
+        "lbmc_set_closure": (ct.c_int, [P, P, ct.c_double, ct.c_double]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -284,6 +307,14 @@ class Lattice:
         self._check(_lib.lbmc_kinetic_energy(self._h, ct.byref(e)))
         return e.value
 
+    # -- learned closure
+    def set_closure(self, model, tau_lo=0.5005, tau_hi=2.0):
+        """Relax plain fluid nodes with the model's tau (Smagorinsky outside
+        [tau_lo, tau_hi]); model=None turns the closure off."""
+        self._closure_model = model
+        h = model._h if model is not None else None
+        self._check(_lib.lbmc_set_closure(self._h, h, float(tau_lo), float(tau_hi)))
+
 
 def copy_bandwidth(nbytes, reps=20, device=0):
     """Time a plain device copy of nbytes (read nbytes + write nbytes) reps times; ms total."""
@@ -292,3 +323,101 @@ def copy_bandwidth(nbytes, reps=20, device=0):
     if st != 0:
         raise LbmcError(st, _lib.lbmc_last_error(None).decode())
     return float(ms.value)
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ct.POINTER(ct.c_float))
+
+
+def _f32(a, shape_tail=()):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+class Mlp:
+    """The closure network on one device: FP32 master weights, Adam state and
+    the bf16 operand image the step and training kernels read.
+
+    params: LBMC_MLP_NPARAM floats (w1[64][8] with the bias in column 7,
+    w2[64][64], b2[64], w3[64], b3).  Inputs are normalised as
+    (feature - mu) * isd; the closure's tau is out_a * y + out_b."""
+
+    def __init__(self, params, mu=None, isd=None, out_a=1.0, out_b=0.0, device=0):
+        p = _f32(params).reshape(-1)
+        if p.size != MLP_NPARAM:
+            raise ValueError(f"{p.size} parameters, expected {MLP_NPARAM}")
+        self.device = device
+        self._h = ct.c_void_p()
+        sc = self._scaling(mu, isd, out_a, out_b)
+        st = _lib.lbmc_mlp_create(int(device), _fptr(p), ct.byref(sc), ct.byref(self._h))
+        if st != 0:
+            raise LbmcError(st, _lib.lbmc_mlp_last_error(None).decode())
+
+    @staticmethod
+    def _scaling(mu, isd, out_a, out_b):
+        sc = MlpScaling()
+        mu = np.zeros(7) if mu is None else np.asarray(mu, dtype=np.float64)
+        isd = np.ones(7) if isd is None else np.asarray(isd, dtype=np.float64)
+        for k in range(7):
+            sc.mu[k] = float(mu[k])
+            sc.isd[k] = float(isd[k])
+        sc.out_a = float(out_a)
+        sc.out_b = float(out_b)
+        return sc
+
+    def _check(self, st):
+        if st != 0:
+            raise LbmcError(st, _lib.lbmc_mlp_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            _lib.lbmc_mlp_destroy(self._h)
+            self._h = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_scaling(self, mu=None, isd=None, out_a=1.0, out_b=0.0):
+        sc = self._scaling(mu, isd, out_a, out_b)
+        self._check(_lib.lbmc_mlp_set_scaling(self._h, ct.byref(sc)))
+
+    def set_params(self, params):
+        p = _f32(params).reshape(-1)
+        if p.size != MLP_NPARAM:
+            raise ValueError(f"{p.size} parameters, expected {MLP_NPARAM}")
+        self._check(_lib.lbmc_mlp_set_params(self._h, _fptr(p)))
+
+    def get_params(self):
+        p = np.empty(MLP_NPARAM, dtype=np.float32)
+        self._check(_lib.lbmc_mlp_get_params(self._h, _fptr(p)))
+        return p
+
+    def forward(self, feat):
+        x = _f32(feat).reshape(-1, 7)
+        y = np.empty(x.shape[0], dtype=np.float32)
+        self._check(_lib.lbmc_mlp_forward(self._h, _fptr(x), x.shape[0], _fptr(y)))
+        return y
+
+    def loss_grad(self, feat, target):
+        x = _f32(feat).reshape(-1, 7)
+        t = _f32(target).reshape(-1)
+        g = np.empty(MLP_NPARAM, dtype=np.float32)
+        loss = ct.c_double()
+        self._check(_lib.lbmc_mlp_loss_grad(self._h, _fptr(x), _fptr(t), x.shape[0], _fptr(g),
+                                            ct.byref(loss)))
+        return loss.value, g
+
+    def set_data(self, feat, target):
+        x = _f32(feat).reshape(-1, 7)
+        t = _f32(target).reshape(-1)
+        if t.size != x.shape[0]:
+            raise ValueError("one target per sample")
+        self._check(_lib.lbmc_mlp_set_data(self._h, _fptr(x), _fptr(t), x.shape[0]))
+
+    def train(self, epochs, batch=16384, lr=1e-3, seed=0):
+        out = np.zeros(max(epochs, 1))
+        self._check(_lib.lbmc_mlp_train(self._h, int(epochs), int(batch), float(lr), int(seed),
+                                        out.ctypes.data_as(ct.POINTER(ct.c_double))))
+        return out[:epochs]
