@@ -64,14 +64,19 @@ def sgs_targets(rho, u, r, tau_c):
     pairs = ((0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (0, 2))
     tsgs = np.stack([box_filter(u[a] * u[b], r) - ub[a] * ub[b] for a, b in pairs])
     S = strain_rate(ub)
+    return rb, ub, S, tsgs, tau_c + 3.0 * eddy_viscosity(tsgs, S)
+
+
+def eddy_viscosity(tsgs, S):
+    """Pointwise least-squares nu_t >= 0 with tau^dev ~ -2 nu_t Sbar (packed
+    symmetric tensors: xx, yy, zz, xy, yz, xz)."""
     tr = (tsgs[0] + tsgs[1] + tsgs[2]) / 3.0
     dev = tsgs.copy()
     dev[0:3] -= tr
-    w = np.array([1, 1, 1, 2, 2, 2], dtype=np.float64)[:, None, None, None]
+    w = np.array([1, 1, 1, 2, 2, 2], dtype=np.float64).reshape((6,) + (1,) * (S.ndim - 1))
     num = -np.sum(w * dev * S, axis=0)
     den = 2.0 * np.sum(w * S * S, axis=0)
-    nu_t = np.maximum(0.0, num / np.maximum(den, 1e-300))
-    return rb, ub, S, tsgs, tau_c + 3.0 * nu_t
+    return np.maximum(0.0, num / np.maximum(den, 1e-300))
 
 
 def features(rho, S, tau_t):

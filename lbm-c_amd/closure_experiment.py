@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Learned relaxation-time closure against Smagorinsky on decaying
+turbulence: trained a priori on a resolved run, tested a posteriori on a
+coarse run from another initial field.
+
+  1. resolved runs on the fine grid (D3Q15 BGK, tau_f, FP32 storage) from
+     the seeded solenoidal field (E(k) ~ k^4 exp(-2 (k/k0)^2)); one seed for
+     training, one for testing;
+  2. training data from the training run's snapshots (closure.sgs_targets,
+     closure.features), subsampled; the network is trained on the GPU by the
+     library's kernels;
+  3. coarse runs (grid / r, tau_c = 1/2 + (tau_f - 1/2) / r, r fine steps per
+     coarse step) from the box-filtered test field, with Smagorinsky
+     (C_s = 0.17) and with the learned closure (Smagorinsky outside
+     [0.5005, 2.0]), and without any model;
+  4. shell-averaged energy spectra of each coarse run against the
+     box-filtered fine test run at the same physical times; error =
+     mean over k in [1, k_max] of |log10(E_coarse / E_ref)|.
+
+Writes one JSON document (--out) and prints progress lines.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+import closure  # noqa: E402
+import lbmc  # noqa: E402
+import workloads  # noqa: E402
+
+
+def log(msg):
+    print(f"[closure-exp {time.strftime('%H:%M:%S')}] {msg}", flush=True)
+
+
+def fine_lattice(L, tau, seed, k0, u_rms, storage):
+    ux, uy, uz = workloads.decaying_turbulence(L, k0=k0, u_rms=u_rms, seed=seed)
+    lat = lbmc.Lattice(lbmc.D3Q15, (L, L, L), lbmc.BGK, tau=tau, storage=storage)
+    lat.set_fields()
+    lat.init_equilibrium(np.ones(L ** 3), (ux.reshape(-1), uy.reshape(-1), uz.reshape(-1)))
+    return lat, (ux, uy, uz)
+
+
+def macros(lat, L):
+    rho, u = lat.get_macros()
+    return rho.reshape(L, L, L), u.reshape(3, L, L, L)
+
+
+def spectrum(u):
+    return workloads.energy_spectrum(u[0], u[1], u[2])
+
+
+def spec_error(e, ref, kmax):
+    k = np.arange(1, kmax + 1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r = np.log10(e[k] / ref[k])
+    if not np.all(np.isfinite(r)):
+        return float("inf")
+    return float(np.mean(np.abs(r)))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--fine", type=int, default=512)
+    ap.add_argument("--ratio", type=int, default=4)
+    ap.add_argument("--tau-fine", type=float, default=0.505)
+    ap.add_argument("--k0", type=float, default=4.0)
+    ap.add_argument("--u-rms", type=float, default=0.05)
+    ap.add_argument("--train-seed", type=int, default=11)
+    ap.add_argument("--test-seed", type=int, default=2024)
+    ap.add_argument("--snapshots", type=str, default="1000,2000,3000,4000,6000,8000",
+                    help="fine steps of the training snapshots")
+    ap.add_argument("--eval", type=str, default="2000,4000,8000", help="fine steps of the comparisons")
+    ap.add_argument("--samples", type=int, default=4_000_000)
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--c-smag", type=float, default=0.17)
+    ap.add_argument("--out", type=str, default="closure_experiment.json")
+    args = ap.parse_args()
+
+    Lf, r = args.fine, args.ratio
+    Lc = Lf // r
+    tau_c = 0.5 + (args.tau_fine - 0.5) / r
+    snaps = [int(s) for s in args.snapshots.split(",")]
+    evals = [int(s) for s in args.eval.split(",")]
+    kmax = Lc // 4
+    res = {"fine": Lf, "coarse": Lc, "ratio": r, "tau_fine": args.tau_fine, "tau_coarse": tau_c,
+           "k0": args.k0, "u_rms": args.u_rms, "c_smag": args.c_smag, "kmax": kmax}
+
+    # ---- 1-2. training data from the resolved training run
+    rng = np.random.default_rng(args.train_seed)
+    per_snap = max(1, args.samples // len(snaps))
+    feats, targs = [], []
+    lat, _ = fine_lattice(Lf, args.tau_fine, args.train_seed, args.k0, args.u_rms, lbmc.FP32)
+    t = 0
+    t0 = time.perf_counter()
+    for ts in snaps:
+        lat.run(t, ts - t, ts - t)
+        t = ts
+        rho, u = macros(lat, Lf)
+        if not (np.all(np.isfinite(u))):
+            log(f"training run not finite at step {ts}")
+            res["error"] = f"training run not finite at step {ts}"
+            break
+        rb, ub, S, tsgs, tau_t = closure.sgs_targets(rho, u, r, tau_c)
+        f = closure.features(rb, S, tau_t)
+        tt = tau_t.reshape(-1).astype(np.float32)
+        pick = rng.choice(f.shape[0], size=min(per_snap, f.shape[0]), replace=False)
+        feats.append(f[pick])
+        targs.append(tt[pick])
+        log(f"train snapshot {ts}: u_rms {np.sqrt(np.mean(u * u)):.4g}, tau' mean {tau_t.mean():.5f} "
+            f"p99 {np.percentile(tau_t, 99):.5f}, share nu_t>0 {np.mean(tau_t > tau_c):.3f}")
+    lat.close()
+    del rho, u
+    res["fine_train_seconds"] = round(time.perf_counter() - t0, 1)
+    if "error" in res:
+        json.dump(res, open(args.out, "w"), indent=1)
+        sys.exit(1)
+    X = np.concatenate(feats)
+    Y = np.concatenate(targs)
+    mu, isd, out_a, out_b = closure.fit_scaling(X, Y)
+    model = lbmc.Mlp(closure.init_params(args.train_seed), mu=mu, isd=isd, out_a=out_a, out_b=out_b)
+    model.set_data(X, (Y - out_b) / out_a)
+    t0 = time.perf_counter()
+    hist = model.train(args.epochs, batch=args.batch, lr=args.lr, seed=args.train_seed)
+    res["train_seconds"] = round(time.perf_counter() - t0, 2)
+    res["train_samples"] = int(X.shape[0])
+    res["train_loss"] = [float(v) for v in hist]
+    res["target_var_normalised"] = 1.0
+    log(f"trained on {X.shape[0]} samples: loss {hist[0]:.4f} -> {hist[-1]:.4f} (normalised target variance 1)")
+
+    # ---- 3. resolved test run: filtered reference spectra and the coarse initial field
+    lat, (ux, uy, uz) = fine_lattice(Lf, args.tau_fine, args.test_seed, args.k0, args.u_rms, lbmc.FP32)
+    ub0 = np.stack([closure.box_filter(v, r) for v in (ux, uy, uz)])
+    ref = {}
+    t = 0
+    for ts in evals:
+        lat.run(t, ts - t, ts - t)
+        t = ts
+        _, u = macros(lat, Lf)
+        ref[ts] = spectrum(np.stack([closure.box_filter(u[a], r) for a in range(3)]))
+        log(f"reference at fine step {ts}: E = {ref[ts].sum():.5g}")
+    lat.close()
+    del u
+
+    # ---- 4. coarse runs
+    runs = {}
+    for name in ("smagorinsky", "learned", "no_model"):
+        c = lbmc.Lattice(lbmc.D3Q15, (Lc, Lc, Lc), lbmc.BGK, tau=tau_c,
+                         c_smag=0.0 if name == "no_model" else args.c_smag, storage=lbmc.FP32)
+        c.set_fields()
+        c.init_equilibrium(np.ones(Lc ** 3), tuple(v.reshape(-1) for v in ub0))
+        if name == "learned":
+            c.set_closure(model, 0.5005, 2.0)
+        errs, tc = {}, 0
+        for ts in evals:
+            n = ts // r
+            c.run(tc, n - tc, n - tc)
+            tc = n
+            _, u = c.get_macros()
+            e = spectrum(u.reshape(3, Lc, Lc, Lc))
+            errs[ts] = {"error": spec_error(e, ref[ts], kmax), "E": float(e.sum()),
+                        "E_ref": float(ref[ts].sum())}
+        # cost per coarse step, same lattice, after the run
+        ms = c.time_steps(tc, 200, 0) / 200
+        c.close()
+        runs[name] = {"by_fine_step": errs, "ms_per_step": round(ms, 5)}
+        log(f"{name}: " + ", ".join(f"t={k}: err {v['error']:.4f}" for k, v in errs.items()) + f", {ms:.4f} ms/step")
+    res["runs"] = runs
+    res["spectra_ref"] = {str(k): v[: kmax + 1].tolist() for k, v in ref.items()}
+    model.close()
+    with open(args.out, "w") as fh:
+        json.dump(res, fh, indent=1)
+    log(f"wrote {args.out}")
+
+
+if __name__ == "__main__":
+    main()
