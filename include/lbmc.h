@@ -224,6 +224,40 @@ lbmc_status lbmc_mlp_train(lbmc_mlp *m, int epochs, int batch, float lr, unsigne
  * weights, or off with model = null. */
 lbmc_status lbmc_set_closure(lbmc_ctx *ctx, const lbmc_mlp *model, double tau_lo, double tau_hi);
 
+/* ---- Batches of lattices ---------------------------------------------------
+ *
+ * Steps an ensemble of independent lattices of different sizes together:
+ * one persistent launch per time step covers every member's nodes, and runs
+ * are replayed from a captured HIP graph.  The reference steps one lattice
+ * per process (LBM-C/index.cu:102-108 calling iterate, :284-313); a batch is
+ * an addition.  Members are contexts made by lbmc_create and set up as usual
+ * (fields and initial state); they share lattice type, storage and device,
+ * may differ in size, collision type, forcing, tau and boundary codes, and
+ * may not use sf boundaries or the learned closure.  Members must outlive
+ * the batch; their state is read at every lbmc_batch_run, and lbmc_get_* on
+ * a member reads what the batch computed.
+ */
+typedef struct lbmc_batch lbmc_batch;
+
+lbmc_status lbmc_batch_create(lbmc_ctx *const *members, int count, lbmc_batch **out);
+lbmc_status lbmc_batch_destroy(lbmc_batch *b);
+const char *lbmc_batch_last_error(const lbmc_batch *b);
+
+/* nsteps calls t0 .. t0+nsteps-1 of every member, macros stored when
+ * (t + 1) % store_every == 0 (never for store_every <= 0); synchronous. */
+lbmc_status lbmc_batch_run(lbmc_batch *b, long long t0, int nsteps, int store_every);
+
+/* The same, timed with HIP events on the batch's stream (ms total). */
+lbmc_status lbmc_batch_time_steps(lbmc_batch *b, long long t0, int nsteps, int store_every, float *ms);
+
+/* Times the persistent kernel's candidate grids (1, 2, 4, 8, 16 blocks per
+ * CU and one block per chunk) over steps_per_trial calls each, starting at
+ * call t0 (the members advance 2 * steps_per_trial calls per candidate),
+ * and keeps the fastest. */
+lbmc_status lbmc_batch_autotune(lbmc_batch *b, long long t0, int steps_per_trial, int *grid, float *ms_per_step);
+lbmc_status lbmc_batch_set_grid(lbmc_batch *b, int grid);
+lbmc_status lbmc_batch_info(const lbmc_batch *b, long long *nodes, int *grid, int *chunks);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,353 @@
+// Batched stepping of several lattices (ensembles of uneven lattices): one
+// persistent launch per time step covers the nodes of every member.
+//
+// The members are ordinary contexts (lbmc_create + lbmc_set_fields + an
+// initial condition).  The batch keeps, in device memory, each member's
+// kernel arguments (its buffers, its collision variant and the slot offsets
+// of the current call t) and a list of 256-node chunks over all members.
+// A step is two launches:
+//   batch_step_kernel     blocks stride over the chunk list; a chunk belongs
+//                         to one member, so the member's arguments and its
+//                         collision variant are uniform over the block;
+//   batch_advance_kernel  one thread per member moves its arguments from
+//                         call t to t + 1 (shift counters, slot offsets and
+//                         the macro-store flag), so the kernels never take t
+//                         as an argument.
+// Because nothing in the launches depends on t, a fixed run of steps is
+// captured once into a HIP graph and replayed for any t.  The grid of the
+// persistent kernel is chosen by lbmc_batch_autotune (timed candidates).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "batch_kernel.hpp"
+#include "ctx.hpp"
+#include "dispatch.hpp"
+
+
+using namespace lbmc;
+
+struct lbmc_batch {
+    int count = 0;
+    int lattice = 0, storage = 0, device = 0;
+    bool cdouble = false;
+    std::vector<lbmc_ctx *> members;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    void *dmembers = nullptr;  // BatchMember<C>[count]
+    int2 *chunks = nullptr;
+    int nchunks = 0;
+    long long nodes = 0;
+    int grid = 1;
+    long long t_next = -1;  // the call the device arguments describe
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_steps = 0;
+    std::string err;
+};
+
+namespace {
+
+std::string g_batch_error;
+
+lbmc_status bfail(lbmc_batch *b, lbmc_status st, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (b) b->err = buf;
+    else g_batch_error = buf;
+    return st;
+}
+
+#define B_HIP(b, call)                                                                              \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess) return bfail((b), LBMC_ERR_DEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+int member_variant(const lbmc_ctx *c)
+{
+    const KernelKey &k = c->key;
+    const int bit0 = k.family == 0 ? (k.smag ? 1 : 0) : (k.mfix ? 1 : 0);
+    return (k.family << 2) | ((k.guo ? 1 : 0) << 1) | bit0;
+}
+
+// Member arguments for call t0, read from the members' current state (so
+// that set_fields / set_f on a member between runs is seen).
+template <class C>
+lbmc_status upload_members(lbmc_batch *b, long long t0, int store_every)
+{
+    std::vector<BatchMember<C>> hm((size_t)b->count);
+    for (int k = 0; k < b->count; k++) {
+        const lbmc_ctx *c = b->members[(size_t)k];
+        if (c->nsf || c->key.closure || c->key.wide || !c->fields_set)
+            return bfail(b, LBMC_ERR_UNSUPPORTED, "batch member %d: sf nodes, closure or state not supported", k);
+        BatchMember<C> &m = hm[(size_t)k];
+        memset(&m, 0, sizeof(m));
+        const int se = store_every > 0 ? store_every : 0;
+        fill_args(b->members[(size_t)k], m.a, t0, se > 0 && ((t0 + 1) % se) == 0);
+        m.variant = member_variant(b->members[(size_t)k]);
+        m.store_every = se;
+        m.store_count = se > 0 ? (int)((t0 + 1) % se) : 0;
+    }
+    B_HIP(b, hipMemcpyAsync(b->dmembers, hm.data(), sizeof(BatchMember<C>) * hm.size(), hipMemcpyHostToDevice,
+                            b->stream));
+    B_HIP(b, hipStreamSynchronize(b->stream));
+    b->t_next = t0;
+    return LBMC_OK;
+}
+
+lbmc_status launch_one_step(lbmc_batch *b)
+{
+    hipError_t e;
+    switch (b->storage) {
+    case LBMC_FP64: e = launch_batch_fp64(b->lattice, b->dmembers, b->chunks, b->nchunks, b->grid, b->stream); break;
+    case LBMC_FP32: e = launch_batch_st<StFP32>(b->lattice, b->dmembers, b->chunks, b->nchunks, b->grid, b->stream); break;
+    case LBMC_FP16S: e = launch_batch_st<StFP16S>(b->lattice, b->dmembers, b->chunks, b->nchunks, b->grid, b->stream); break;
+    default: e = launch_batch_st<StBF16S>(b->lattice, b->dmembers, b->chunks, b->nchunks, b->grid, b->stream); break;
+    }
+    if (e != hipSuccess) return bfail(b, LBMC_ERR_DEVICE, "batch step launch: %s", hipGetErrorString(e));
+    const int nb = (b->count + 63) / 64;
+    if (b->cdouble)
+        hipLaunchKernelGGL(batch_advance_kernel<double>, dim3(nb), dim3(64), 0, b->stream,
+                           (BatchMember<double> *)b->dmembers, b->count);
+    else
+        hipLaunchKernelGGL(batch_advance_kernel<float>, dim3(nb), dim3(64), 0, b->stream,
+                           (BatchMember<float> *)b->dmembers, b->count);
+    e = hipGetLastError();
+    if (e != hipSuccess) return bfail(b, LBMC_ERR_DEVICE, "batch advance launch: %s", hipGetErrorString(e));
+    return LBMC_OK;
+}
+
+void drop_batch_graph(lbmc_batch *b)
+{
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+    b->graph_steps = 0;
+}
+
+constexpr int kBatchGraphSteps = 32;
+
+lbmc_status upload(lbmc_batch *b, long long t0, int store_every)
+{
+    return b->cdouble ? upload_members<double>(b, t0, store_every) : upload_members<float>(b, t0, store_every);
+}
+
+// nsteps calls from t0; the device arguments must describe call t0
+lbmc_status batch_steps(lbmc_batch *b, long long t0, int nsteps)
+{
+    int left = nsteps;
+    if (left >= kBatchGraphSteps) {
+        if (!b->graph_exec) {
+            B_HIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            lbmc_status st = LBMC_OK;
+            for (int k = 0; k < kBatchGraphSteps && st == LBMC_OK; k++) st = launch_one_step(b);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(b->stream, &g);
+            if (st != LBMC_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return st;
+            }
+            if (e != hipSuccess) return bfail(b, LBMC_ERR_DEVICE, "batch graph capture: %s", hipGetErrorString(e));
+            b->graph = g;
+            B_HIP(b, hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
+            b->graph_steps = kBatchGraphSteps;
+        }
+        while (left >= b->graph_steps) {
+            B_HIP(b, hipGraphLaunch(b->graph_exec, b->stream));
+            left -= b->graph_steps;
+        }
+    }
+    for (; left > 0; left--) {
+        lbmc_status st = launch_one_step(b);
+        if (st != LBMC_OK) return st;
+    }
+    b->t_next = t0 + nsteps;
+    return LBMC_OK;
+}
+
+lbmc_status sync_members(lbmc_batch *b)
+{
+    for (lbmc_ctx *c : b->members) B_HIP(b, hipStreamSynchronize(c->stream));
+    return LBMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *lbmc_batch_last_error(const lbmc_batch *b) { return b ? b->err.c_str() : g_batch_error.c_str(); }
+
+lbmc_status lbmc_batch_create(lbmc_ctx *const *members, int count, lbmc_batch **out)
+{
+    if (!out || !members || count <= 0) return bfail(nullptr, LBMC_ERR_INVALID, "lbmc_batch_create: bad argument");
+    *out = nullptr;
+    const lbmc_ctx *c0 = members[0];
+    if (!c0) return bfail(nullptr, LBMC_ERR_INVALID, "lbmc_batch_create: member 0 is null");
+    long long nchunks = 0, nodes = 0;
+    for (int k = 0; k < count; k++) {
+        const lbmc_ctx *c = members[k];
+        if (!c) return bfail(nullptr, LBMC_ERR_INVALID, "lbmc_batch_create: member %d is null", k);
+        if (!c->fields_set) return bfail(nullptr, LBMC_ERR_STATE, "lbmc_batch_create: member %d has no fields", k);
+        if (c->desc.lattice != c0->desc.lattice || c->storage != c0->storage || c->desc.device != c0->desc.device)
+            return bfail(nullptr, LBMC_ERR_INVALID,
+                         "lbmc_batch_create: member %d differs in lattice, storage or device from member 0", k);
+        if (c->nsf) return bfail(nullptr, LBMC_ERR_UNSUPPORTED, "lbmc_batch_create: member %d has sf boundary nodes", k);
+        if (c->key.closure)
+            return bfail(nullptr, LBMC_ERR_UNSUPPORTED, "lbmc_batch_create: member %d uses the learned closure", k);
+        if (c->key.wide) return bfail(nullptr, LBMC_ERR_UNSUPPORTED, "lbmc_batch_create: member %d is too large", k);
+        for (int j = 0; j < k; j++)
+            if (members[j] == c) return bfail(nullptr, LBMC_ERR_INVALID, "lbmc_batch_create: member %d repeats %d", k, j);
+        nchunks += (c->N + kBlock - 1) / kBlock;
+        nodes += c->N;
+    }
+    if (nchunks >= (1LL << 31)) return bfail(nullptr, LBMC_ERR_UNSUPPORTED, "lbmc_batch_create: too many nodes");
+    lbmc_batch *b = new lbmc_batch();
+    b->count = count;
+    b->lattice = c0->desc.lattice;
+    b->storage = c0->storage;
+    b->device = c0->desc.device;
+    b->cdouble = c0->cbytes == 8;
+    b->members.assign(members, members + count);
+    b->nchunks = (int)nchunks;
+    b->nodes = nodes;
+    auto bail = [&](lbmc_status st) {
+        g_batch_error = b->err;
+        lbmc_batch_destroy(b);
+        return st;
+    };
+    std::vector<int2> ch;
+    ch.reserve((size_t)nchunks);
+    for (int k = 0; k < count; k++)
+        for (long long n = 0; n < members[k]->N; n += kBlock) ch.push_back(make_int2(k, (int)n));
+    hipError_t e = hipSetDevice(b->device);
+    if (e == hipSuccess) e = hipStreamCreate(&b->stream);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    const size_t msz = b->cdouble ? sizeof(BatchMember<double>) : sizeof(BatchMember<float>);
+    if (e == hipSuccess) e = hipMalloc(&b->dmembers, msz * (size_t)count);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->chunks, sizeof(int2) * ch.size());
+    if (e == hipSuccess) e = hipMemcpy(b->chunks, ch.data(), sizeof(int2) * ch.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return bail(bfail(b, LBMC_ERR_DEVICE, "lbmc_batch_create: %s", hipGetErrorString(e)));
+    // default persistent grid: 8 blocks per CU, at most one block per chunk
+    int ncu = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, b->device) == hipSuccess && prop.multiProcessorCount > 0)
+        ncu = prop.multiProcessorCount;
+    b->grid = (int)std::min<long long>(nchunks, 8LL * ncu);
+    *out = b;
+    return LBMC_OK;
+}
+
+lbmc_status lbmc_batch_destroy(lbmc_batch *b)
+{
+    if (!b) return LBMC_OK;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_batch_graph(b);
+    if (b->dmembers) (void)hipFree(b->dmembers);
+    if (b->chunks) (void)hipFree(b->chunks);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return LBMC_OK;
+}
+
+lbmc_status lbmc_batch_run(lbmc_batch *b, long long t0, int nsteps, int store_every)
+{
+    if (!b || nsteps < 0 || t0 < 0) return bfail(b, LBMC_ERR_INVALID, "lbmc_batch_run: bad argument");
+    B_HIP(b, hipSetDevice(b->device));
+    lbmc_status st = sync_members(b);
+    if (st == LBMC_OK) st = upload(b, t0, store_every);
+    if (st == LBMC_OK) st = batch_steps(b, t0, nsteps);
+    if (st != LBMC_OK) return st;
+    B_HIP(b, hipStreamSynchronize(b->stream));
+    return LBMC_OK;
+}
+
+lbmc_status lbmc_batch_time_steps(lbmc_batch *b, long long t0, int nsteps, int store_every, float *ms)
+{
+    if (!b || nsteps <= 0 || t0 < 0 || !ms) return bfail(b, LBMC_ERR_INVALID, "lbmc_batch_time_steps: bad argument");
+    B_HIP(b, hipSetDevice(b->device));
+    lbmc_status st = sync_members(b);
+    if (st == LBMC_OK) st = upload(b, t0, store_every);  // outside the timed region
+    if (st != LBMC_OK) return st;
+    B_HIP(b, hipEventRecord(b->ev0, b->stream));
+    st = batch_steps(b, t0, nsteps);
+    if (st != LBMC_OK) return st;
+    B_HIP(b, hipEventRecord(b->ev1, b->stream));
+    B_HIP(b, hipEventSynchronize(b->ev1));
+    B_HIP(b, hipEventElapsedTime(ms, b->ev0, b->ev1));
+    return LBMC_OK;
+}
+
+// Times candidate persistent grids (blocks per CU 1..16, and one block per
+// chunk) over a few steps from the current state and keeps the fastest.
+// The steps advance the members' state; callers tune before the timed run.
+lbmc_status lbmc_batch_autotune(lbmc_batch *b, long long t0, int steps_per_trial, int *grid, float *ms_per_step)
+{
+    if (!b || steps_per_trial <= 0 || t0 < 0) return bfail(b, LBMC_ERR_INVALID, "lbmc_batch_autotune: bad argument");
+    B_HIP(b, hipSetDevice(b->device));
+    int ncu = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, b->device) == hipSuccess && prop.multiProcessorCount > 0)
+        ncu = prop.multiProcessorCount;
+    std::vector<int> cand;
+    for (int per : {1, 2, 4, 8, 16}) cand.push_back((int)std::min<long long>(b->nchunks, (long long)per * ncu));
+    cand.push_back(b->nchunks);
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    float best = -1.f;
+    int best_grid = b->grid;
+    long long t = t0;
+    for (int g : cand) {
+        drop_batch_graph(b);
+        b->grid = g;
+        float ms = 0.f;
+        // one untimed pass (graph build), then the timed one
+        lbmc_status st = lbmc_batch_time_steps(b, t, steps_per_trial, 0, &ms);
+        if (st != LBMC_OK) return st;
+        t += steps_per_trial;
+        st = lbmc_batch_time_steps(b, t, steps_per_trial, 0, &ms);
+        if (st != LBMC_OK) return st;
+        t += steps_per_trial;
+        if (best < 0.f || ms < best) {
+            best = ms;
+            best_grid = g;
+        }
+    }
+    drop_batch_graph(b);
+    b->grid = best_grid;
+    if (grid) *grid = best_grid;
+    if (ms_per_step) *ms_per_step = best / (float)steps_per_trial;
+    return LBMC_OK;
+}
+
+lbmc_status lbmc_batch_info(const lbmc_batch *b, long long *nodes, int *grid, int *chunks)
+{
+    if (!b) return LBMC_ERR_INVALID;
+    if (nodes) *nodes = b->nodes;
+    if (grid) *grid = b->grid;
+    if (chunks) *chunks = b->nchunks;
+    return LBMC_OK;
+}
+
+lbmc_status lbmc_batch_set_grid(lbmc_batch *b, int grid)
+{
+    if (!b || grid <= 0) return bfail(b, LBMC_ERR_INVALID, "lbmc_batch_set_grid: grid must be positive");
+    drop_batch_graph(b);
+    b->grid = std::min(grid, b->nchunks);
+    return LBMC_OK;
+}
+
+}  // extern "C"

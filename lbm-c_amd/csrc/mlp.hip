@@ -441,12 +441,12 @@ lbmc_status lbmc_mlp_create(int device, const float *params, const lbmc_mlp_scal
 void lbmc_mlp_destroy(lbmc_mlp *m)
 {
     if (!m) return;
-    hipSetDevice(m->device);
-    if (m->stream) hipStreamSynchronize(m->stream);
+    (void)hipSetDevice(m->device);
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
     void *bufs[] = {m->params, m->m1, m->m2, m->grad, m->img, m->feat, m->target, m->order, m->partial, m->lossacc};
     for (void *b : bufs)
-        if (b) hipFree(b);
-    if (m->stream) hipStreamDestroy(m->stream);
+        if (b) (void)hipFree(b);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
 }
 
@@ -501,8 +501,8 @@ lbmc_status lbmc_mlp_forward(lbmc_mlp *m, const float *feat, long long S, float 
     }
     if (e == hipSuccess) e = hipMemcpyAsync(y, dy, (size_t)S * sizeof(float), hipMemcpyDeviceToHost, m->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
-    hipFree(dx);
-    if (dy) hipFree(dy);
+    (void)hipFree(dx);
+    if (dy) (void)hipFree(dy);
     if (e != hipSuccess) return mfail(m, LBMC_ERR_DEVICE, "lbmc_mlp_forward: %s", hipGetErrorString(e));
     return LBMC_OK;
 }

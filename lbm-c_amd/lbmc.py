@@ -39,6 +39,8 @@ EXPORTS = [
     "lbmc_mlp_create", "lbmc_mlp_destroy", "lbmc_mlp_last_error", "lbmc_mlp_set_params",
     "lbmc_mlp_get_params", "lbmc_mlp_set_scaling", "lbmc_mlp_forward", "lbmc_mlp_loss_grad",
     "lbmc_mlp_set_data", "lbmc_mlp_train", "lbmc_set_closure",
+    "lbmc_batch_create", "lbmc_batch_destroy", "lbmc_batch_last_error", "lbmc_batch_run",
+    "lbmc_batch_time_steps", "lbmc_batch_autotune", "lbmc_batch_set_grid", "lbmc_batch_info",
 ]
 
 MLP_NPARAM = 4737
@@ -118,6 +120,15 @@ def _load():
         "lbmc_mlp_train": (ct.c_int, [P, ct.c_int, ct.c_int, ct.c_float, ct.c_ulonglong,
                                       ct.POINTER(ct.c_double)]),
         "lbmc_set_closure": (ct.c_int, [P, P, ct.c_double, ct.c_double]),
+        "lbmc_batch_create": (ct.c_int, [ct.POINTER(P), ct.c_int, ct.POINTER(P)]),
+        "lbmc_batch_destroy": (ct.c_int, [P]),
+        "lbmc_batch_last_error": (ct.c_char_p, [P]),
+        "lbmc_batch_run": (ct.c_int, [P, ct.c_longlong, ct.c_int, ct.c_int]),
+        "lbmc_batch_time_steps": (ct.c_int, [P, ct.c_longlong, ct.c_int, ct.c_int, fp]),
+        "lbmc_batch_autotune": (ct.c_int, [P, ct.c_longlong, ct.c_int, ct.POINTER(ct.c_int), fp]),
+        "lbmc_batch_set_grid": (ct.c_int, [P, ct.c_int]),
+        "lbmc_batch_info": (ct.c_int, [P, ct.POINTER(ct.c_longlong), ct.POINTER(ct.c_int),
+                                       ct.POINTER(ct.c_int)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -421,3 +432,58 @@ class Mlp:
         self._check(_lib.lbmc_mlp_train(self._h, int(epochs), int(batch), float(lr), int(seed),
                                         out.ctypes.data_as(ct.POINTER(ct.c_double))))
         return out[:epochs]
+
+
+class Batch:
+    """Several Lattice objects (same lattice type, storage and device) stepped
+    together by one persistent launch per step.  The members keep their own
+    read-back methods; they must stay open while the batch is used."""
+
+    def __init__(self, members):
+        self.members = list(members)
+        arr = (ct.c_void_p * len(self.members))(*[m._h.value for m in self.members])
+        self._h = ct.c_void_p()
+        st = _lib.lbmc_batch_create(arr, len(self.members), ct.byref(self._h))
+        if st != 0:
+            raise LbmcError(st, _lib.lbmc_batch_last_error(None).decode())
+
+    def _check(self, st):
+        if st != 0:
+            raise LbmcError(st, _lib.lbmc_batch_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            _lib.lbmc_batch_destroy(self._h)
+            self._h = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, t0, nsteps, store_every=1):
+        self._check(_lib.lbmc_batch_run(self._h, int(t0), int(nsteps), int(store_every)))
+
+    def time_steps(self, t0, nsteps, store_every=0):
+        ms = ct.c_float()
+        self._check(_lib.lbmc_batch_time_steps(self._h, int(t0), int(nsteps), int(store_every),
+                                               ct.byref(ms)))
+        return float(ms.value)
+
+    def autotune(self, t0, steps_per_trial=20):
+        g = ct.c_int()
+        ms = ct.c_float()
+        self._check(_lib.lbmc_batch_autotune(self._h, int(t0), int(steps_per_trial), ct.byref(g),
+                                             ct.byref(ms)))
+        return g.value, float(ms.value)
+
+    def set_grid(self, grid):
+        self._check(_lib.lbmc_batch_set_grid(self._h, int(grid)))
+
+    def info(self):
+        n = ct.c_longlong()
+        g = ct.c_int()
+        c = ct.c_int()
+        self._check(_lib.lbmc_batch_info(self._h, ct.byref(n), ct.byref(g), ct.byref(c)))
+        return {"nodes": n.value, "grid": g.value, "chunks": c.value}

@@ -1,0 +1,118 @@
+"""Batched stepping (lbmc_batch_*): every member of a batch must end where
+the same lattice stepped on its own ends -- bit for bit against the FP64
+oracle with FP64 storage (each member a different collision code, boundary
+mix and size), and against the single-lattice kernels with FP32 storage;
+also from a large t0 (index-shift wrap), across graph replays, and with the
+autotuned grid."""
+import numpy as np
+import pytest
+
+import cases
+import lbmc
+import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _members(lattice):
+    out = []
+    for c in cases.all_cases():
+        if c["desc"]["lattice"] != lattice or "sf" in c["name"]:
+            continue
+        out.append(c)
+    return out
+
+
+def _run_batch(group, storage, t0, steps, store_every=1, grid=None):
+    lats = [cases.build(lbmc.Lattice, c, storage=storage) for c in group]
+    b = lbmc.Batch(lats)
+    if grid is not None:
+        b.set_grid(grid)
+    b.run(t0, steps, store_every)
+    out = [(lat.get_f(), lat.get_macros()) for lat in lats]
+    b.close()
+    for lat in lats:
+        lat.close()
+    return out
+
+
+def _oracle(group, t0, steps, store_every=1):
+    out = []
+    for c in group:
+        o = cases.build(pyoracle.Oracle, c)
+        o.run(t0, steps, store_every)
+        out.append((o.get_f(), o.get_macros()))
+        o.close()
+    return out
+
+
+@pytest.mark.parametrize("lattice", [lbmc.D2Q9, lbmc.D3Q15])
+@pytest.mark.parametrize("t0", [0, 2 ** 31 + 5])
+def test_batch_fp64_matches_oracle_bitwise(lattice, t0):
+    group = _members(lattice)
+    print(f"[batch] FP64 lattice={lattice} t0={t0} members={len(group)}", flush=True)
+    steps = 45  # one graph replay of 32 calls plus 13 single launches
+    got = _run_batch(group, lbmc.FP64, t0, steps)
+    ref = _oracle(group, t0, steps)
+    for c, (fg, (rg, ug)), (fr, (rr, ur)) in zip(group, got, ref):
+        assert np.array_equal(fg, fr), c["name"]
+        assert np.array_equal(rg, rr), c["name"]
+        assert np.array_equal(ug, ur), c["name"]
+
+
+def test_batch_fp32_matches_single_lattices():
+    group = _members(lbmc.D2Q9)
+    print(f"[batch] FP32 members={len(group)}", flush=True)
+    steps = 70
+    got = _run_batch(group, lbmc.FP32, 0, steps, store_every=7)
+    for c, (fg, (rg, ug)) in zip(group, got):
+        lat = cases.build(lbmc.Lattice, c, storage=lbmc.FP32)
+        lat.run(0, steps, 7)
+        fs = lat.get_f()
+        rs, us = lat.get_macros()
+        lat.close()
+        assert np.max(np.abs(fg - fs)) <= 1e-6, c["name"]
+        assert np.max(np.abs(ug - us)) <= 1e-6, c["name"]
+
+
+def test_batch_store_schedule():
+    # store_every = 10 over 25 calls: the stored macros are those of call 19
+    group = _members(lbmc.D2Q9)[:3]
+    got = _run_batch(group, lbmc.FP64, 0, 25, store_every=10)
+    for c, (_, (rg, ug)) in zip(group, got):
+        o = cases.build(pyoracle.Oracle, c)
+        o.run(0, 20, 10)
+        rr, ur = o.get_macros()
+        o.close()
+        assert np.array_equal(rg, rr) and np.array_equal(ug, ur), c["name"]
+
+
+def test_autotune_then_run_matches_oracle():
+    group = _members(lbmc.D2Q9)
+    lats = [cases.build(lbmc.Lattice, c, storage=lbmc.FP64) for c in group]
+    b = lbmc.Batch(lats)
+    info = b.info()
+    grid, ms = b.autotune(0, steps_per_trial=8)
+    assert 1 <= grid <= info["chunks"] and ms > 0
+    b.close()
+    for lat in lats:
+        lat.close()
+    got = _run_batch(group, lbmc.FP64, 0, 40, grid=grid)
+    ref = _oracle(group, 0, 40)
+    for c, (fg, _), (fr, _) in zip(group, got, ref):
+        assert np.array_equal(fg, fr), c["name"]
+
+
+def test_batch_rejects_mixed_members():
+    a = lbmc.Lattice(lbmc.D2Q9, (8, 8), lbmc.BGK, tau=0.8, storage=lbmc.FP32)
+    b = lbmc.Lattice(lbmc.D2Q9, (8, 8), lbmc.BGK, tau=0.8, storage=lbmc.FP64)
+    a.set_fields()
+    b.set_fields()
+    with pytest.raises(lbmc.LbmcError):
+        lbmc.Batch([a, b])
+    c = lbmc.Lattice(lbmc.D2Q9, (8, 8), lbmc.BGK, tau=0.8, storage=lbmc.FP32)
+    with pytest.raises(lbmc.LbmcError) as e:
+        lbmc.Batch([a, c])  # c has no fields yet
+    assert e.value.status == 4
+    for x in (a, b, c):
+        x.close()
