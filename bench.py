@@ -80,6 +80,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-variants", action="store_true", help="skip the FP16S/BF16S lines")
     ap.add_argument("--no-closure", action="store_true", help="skip the learned-closure step line")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble line")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     args = ap.parse_args()
 
@@ -172,6 +173,35 @@ def main():
                         "mfma_flop_per_node": 2 * (16 * 64 + 64 * 64 + 64 * 16)}
         log(f"bench: closure {closure_line}")
 
+    ensemble_line = None
+    if not args.no_ensemble:
+        # 64 uneven D2Q9 channels with discs (seeded), FP32, one persistent
+        # batched launch per step, autotuned grid
+        specs = workloads.ensemble_d2q9(64, 0)
+        EN = sum(m["length"][0] * m["length"][1] for m in specs)
+        lats = []
+        for m in specs:
+            Lx, Ly = m["length"]
+            lt = lbmc.Lattice(lbmc.D2Q9, (Lx, Ly), lbmc.MRT if m["collision"] == "MRT" else lbmc.BGK,
+                              forcing=True, tau=m["tau"], tau_mrt=m["rates"], storage=lbmc.FP32)
+            lt.set_fields(geometry=m["geometry"].reshape(-1),
+                          force=[np.full(Lx * Ly, m["force"]), np.zeros(Lx * Ly)])
+            lt.init_static()
+            lats.append(lt)
+        b = lbmc.Batch(lats)
+        grid, _ = b.autotune(0, steps_per_trial=20)
+        esteps = max(64, args.steps)
+        b.time_steps(1000, 64, 0)
+        ems = b.time_steps(1064, esteps, 0)
+        b.close()
+        for lt in lats:
+            lt.close()
+        ensemble_line = {"members": 64, "nodes": EN, "MLUPS": round(EN * esteps / (ems * 1e-3) / 1e6, 1),
+                         "ms_per_step": round(ems / esteps, 5), "grid": grid,
+                         "bytes_per_node": 2 * 9 * 4 + 1,
+                         "achieved_GBs": round(73 * EN / (ems / esteps * 1e-3) / 1e9, 1)}
+        log(f"bench: ensemble {ensemble_line}")
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -203,6 +233,7 @@ def main():
         "cpu_baseline": cpu,
         "variants": variants,
         "closure": closure_line,
+        "ensemble": ensemble_line,
         "finite": finite,
     }
     print(json.dumps(out), flush=True)

@@ -7,6 +7,9 @@
   porous_discs         non-overlapping discs with radii in [rmin, rmax] placed
                        until a solid fraction is reached, walls top/bottom,
                        inlet/outlet columns kept clear (the D2Q9 porous case)
+  ensemble_d2q9        seeded ensemble of uneven D2Q9 channels with 1-6 discs
+                       each, Reynolds number set through tau and the body
+                       force (the batched-ensemble case)
 
 All generators take an explicit seed and use numpy's PCG64 generator, so the
 fields (and the node masks built from them) are reproducible bit for bit.
@@ -86,3 +89,53 @@ def porous_discs(Lx, Ly, rmin=8, rmax=24, solid_fraction=0.30, seed=0, margin=No
         sub = (xx[:, x0:x1] - cx) ** 2 + (yy[y0:y1, :] - cy) ** 2 <= r * r
         geo[y0:y1, x0:x1][sub] = 1.0
     return geo, discs
+
+
+def ensemble_d2q9(count=64, seed=0, nx=(128, 1024), ny=(64, 512), re=(10.0, 1000.0), u_mean=0.04,
+                  tau_min=0.505):
+    """Member specifications of a batched ensemble: periodic-in-x channels
+    (solid rows y = 0 and y = ny-1) of seeded size, with 1-6 seeded
+    non-overlapping discs of radius in [ny/16, ny/8], driven by a uniform
+    body force.  Re = u_mean D / nu with D the mean disc diameter, log-uniform
+    in `re`; nu = u_mean D / Re gives tau = 1/2 + 3 nu (raised to tau_min, in
+    which case u_mean drops to keep Re), and the force follows the plane
+    Poiseuille estimate F = 12 nu u_mean / H^2.  Even members use MRT with
+    the shear and momentum rates 1/tau (the consistent Guo forcing), odd
+    members BGK."""
+    rng = np.random.default_rng(seed)
+    members = []
+    for k in range(count):
+        Lx = int(rng.integers(nx[0] // 16, nx[1] // 16 + 1)) * 16
+        Ly = int(rng.integers(ny[0] // 16, ny[1] // 16 + 1)) * 16
+        geo = channel(Lx, Ly)
+        yy, xx = np.ogrid[0:Ly, 0:Lx]
+        ndisc = int(rng.integers(1, 7))
+        discs = []
+        tries = 0
+        while len(discs) < ndisc and tries < 10000:
+            tries += 1
+            r = rng.uniform(Ly / 16.0, Ly / 8.0)
+            cx = rng.uniform(r + 1, Lx - r - 2)
+            cy = rng.uniform(r + 3, Ly - r - 4)
+            if any((cx - x) ** 2 + (cy - y) ** 2 < (r + s + 2) ** 2 for x, y, s in discs):
+                continue
+            discs.append((cx, cy, r))
+            geo[(xx - cx) ** 2 + (yy - cy) ** 2 <= r * r] = 1.0
+        D = 2.0 * float(np.mean([d[2] for d in discs]))
+        Re = float(np.exp(rng.uniform(np.log(re[0]), np.log(re[1]))))
+        nu = u_mean * D / Re
+        tau = 0.5 + 3.0 * nu
+        u = u_mean
+        if tau < tau_min:
+            tau = tau_min
+            nu = (tau - 0.5) / 3.0
+            u = Re * nu / D
+        H = Ly - 2
+        F = 12.0 * nu * u / (H * H)
+        mrt = k % 2 == 0
+        s = 1.0 / tau
+        members.append(dict(
+            name=f"member{k}", length=(Lx, Ly), geometry=geo, discs=discs, Re=Re, tau=tau, force=F,
+            collision="MRT" if mrt else "BGK",
+            rates=[0.0, 1.64, 1.54, s, 1.9, s, 1.9, s, s] if mrt else None))
+    return members
