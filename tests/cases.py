@@ -183,6 +183,31 @@ def nonuniform_force_d3q15(L=(8, 6, 5), steps=20, seed=7):
         init=("equilibrium", np.ones(n), u), steps=steps)
 
 
+def porous_media_d2q9(Lx=2048, Ly=1024, tau=0.6, rho_in=1.001, rho_out=1.000, seed=0, steps=0):
+    """The porous-medium configuration: seeded discs (radius 8..24) to a solid
+    fraction of 0.30 between solid walls, Zou-He pressure inlet (x = 0) and
+    outlet (x = Lx-1) with macro code 8, MRT with Lallemand-Luo rates."""
+    import workloads
+
+    geo, _ = workloads.porous_discs(Lx, Ly, 8, 24, 0.30, seed=seed)
+    macro = np.zeros((Ly, Lx), dtype=np.int32)
+    micro = np.zeros((Ly, Lx), dtype=np.int32)
+    macro[1:-1, 0] = 8
+    macro[1:-1, -1] = 8
+    micro[1:-1, 0] = 1
+    micro[1:-1, -1] = 2
+    rho = np.ones((Ly, Lx))
+    rho[:, 0] = rho_in
+    rho[:, -1] = rho_out
+    return dict(
+        name=f"porous_media_d2q9_{Lx}x{Ly}",
+        desc=dict(lattice=D2Q9, length=(Lx, Ly), collision_type=MRT, macro_bc=True, micro_bc=True,
+                  tau=tau, tau_mrt=d2q9_rates(tau)),
+        fields=dict(geometry=geo.reshape(-1), macro_bc=macro.reshape(-1), micro_bc=micro.reshape(-1),
+                    rho=rho.reshape(-1)),
+        init="static", steps=steps)
+
+
 def all_cases():
     """The parity matrix: every collision code, BC kind and quirk flag."""
     out = [

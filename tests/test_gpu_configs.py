@@ -68,6 +68,35 @@ def test_c1_poiseuille_channel():
         lat.close()
 
 
+def test_c2_porous_media_full_size():
+    """D2Q9 MRT porous medium, 2048x1024 with Zou-He pressure ends: FP64
+    storage bit-identical to the oracle over 60 calls (graph replays
+    included: lcm(2048, 1024) is too long to capture, so these are direct
+    launches); FP32 within the FP32 bound."""
+    case = cases.porous_media_d2q9()
+    steps = 60
+    print("[c2] oracle 2048x1024, 60 steps", flush=True)
+    ref = cases.build(pyoracle.Oracle, case)
+    ref.run(0, steps, steps)
+    fo = ref.get_f()
+    ro, uo = ref.get_macros()
+    ref.close()
+    for storage in (lbmc.FP64, lbmc.FP32):
+        print(f"[c2] GPU storage {storage}", flush=True)
+        lat = cases.build(lbmc.Lattice, case, storage=storage)
+        lat.run(0, steps, steps)
+        f = lat.get_f()
+        rho, u = lat.get_macros()
+        lat.close()
+        if storage == lbmc.FP64:
+            assert np.array_equal(f, fo)
+            assert np.array_equal(rho, ro) and np.array_equal(u, uo)
+        else:
+            K = 64 * 2.0 ** -24 * np.sqrt(steps)
+            assert np.max(np.abs(f - fo)) <= K * np.max(np.abs(fo))
+            assert np.max(np.abs(u - uo)) <= K
+
+
 @pytest.mark.parametrize("storage", [0, 1], ids=["fp64", "fp32"])
 def test_graph_replay_matches_direct(storage, monkeypatch):
     case = cases.pressure_d2q9(Lx=24, Ly=12, collision=cases.MRT, steps=0)
