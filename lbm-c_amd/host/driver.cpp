@@ -6,8 +6,9 @@
 // Per step i = 1..MaxT it calls the step with time index i-1
 // (index.cu:108), then, as the reference:
 //   * every FileOut steps appends VelocityX/Y/Z and Density (as selected by
-//     OutputVars) to OutputFile (output_macros, index.cu:196-281; a field
-//     file instead of CGNS, see field_file.hpp);
+//     OutputVars) to OutputFile (output_macros, index.cu:196-281): a CGNS
+//     FlowSolution when the name ends in ".cgns", field-file records
+//     otherwise (field_file.hpp);
 //   * every ScreenMes steps prints the ScreenNode's rho, u and residual
 //     (screen_mess, index.cu:412-439);
 //   * every SteadyCheck steps computes the kinetic-energy residual
@@ -38,6 +39,7 @@ namespace {
 struct Driver {
     ModelBuilder &mb;
     double prev_energy = 0;
+    SolutionSink sink;
 
     // output_macros, index.cu:196-281
     void output_macros(long long time)
@@ -49,22 +51,21 @@ struct Driver {
         lbmc_safe(mb.ctx, lbmc_get_macros(mb.ctx, rho.data(), up), "Output: Copy from device memory failed!");
         std::vector<FieldRecord> recs;
         const char *labels[3] = {"VelocityX", "VelocityY", "VelocityZ"};
-        const std::string at = "@" + std::to_string(time);
         for (int d = 0; d < mb.dim(); d++) {
             if (!mb.oc.u[d]) continue;
             FieldRecord r;
-            r.name = labels[d] + at;
+            r.name = labels[d];
             r.f64 = u[d];
             recs.push_back(std::move(r));
         }
         if (mb.oc.rho) {
             FieldRecord r;
-            r.name = "Density" + at;
+            r.name = "Density";
             r.f64 = rho;
             recs.push_back(std::move(r));
         }
         std::string err;
-        if (!FieldFile::append(mb.project.output_fname, recs, &err)) {
+        if (!mb.project.output_fname.empty() && !sink.append(time, recs, &err)) {
             fprintf(stderr, "ERROR: Output Handler: %s\n", err.c_str());
             exit(-1);
         }
@@ -114,18 +115,14 @@ int main(int argc, char **argv)
     }
 
     ModelBuilder mb(argv[1], std::cout);
-    Driver drv{mb};
+    Driver drv{mb, 0.0, SolutionSink()};
     {
-        // the output file starts with the grid dimensions (the reference
-        // writes base and grid when it creates the CGNS file)
-        FieldFile out;
-        FieldRecord dims;
-        dims.name = "Dims";
-        dims.type = 2;
-        dims.i32 = {mb.dc.length[0], mb.dc.length[1], mb.dim() > 2 ? mb.dc.length[2] : 1};
-        out.records.push_back(dims);
+        // the output starts with the grid (the reference writes base, zone
+        // and coordinates when it creates the CGNS file, index.cu:184)
         std::string err;
-        if (!mb.project.output_fname.empty() && !out.write(mb.project.output_fname, &err)) {
+        if (!mb.project.output_fname.empty() &&
+            !drv.sink.create(mb.project.output_fname, mb.dc.length[0], mb.dc.length[1],
+                             mb.dim() > 2 ? mb.dc.length[2] : 1, &err)) {
             fprintf(stderr, "ERROR: Output Handler: %s\n", err.c_str());
             return -1;
         }
@@ -190,9 +187,10 @@ int main(int argc, char **argv)
     if (!checkpoint.empty()) {
         // restartable with Init 2: the input fields plus f0..f<Q-1> and TimeStep
         mb.write_checkpoint(checkpoint + ".tmp", t0 + i - 1);
-        FieldFile in, ck;
+        FieldFile ck;
+        const FieldFile &in = mb.inputs;
         std::string err;
-        if (!in.read(mb.project.domain_fname, &err) || !ck.read(checkpoint + ".tmp", &err)) {
+        if (!ck.read(checkpoint + ".tmp", &err)) {
             fprintf(stderr, "ERROR: Checkpoint: %s\n", err.c_str());
             return -1;
         }

@@ -1,6 +1,10 @@
 // LBMCFLD1 field container (format in field_file.hpp).
 #include "field_file.hpp"
 
+#include <dlfcn.h>
+#include <unistd.h>
+
+#include <climits>
 #include <cstdio>
 #include <cstring>
 
@@ -113,6 +117,157 @@ const FieldRecord *FieldFile::find(const std::string &name) const
     for (const FieldRecord &r : records)
         if (r.name == name) return &r;
     return nullptr;
+}
+
+bool is_cgns(const std::string &fname)
+{
+    return fname.size() >= 5 && fname.compare(fname.size() - 5, 5, ".cgns") == 0;
+}
+
+namespace {
+
+// the C entry points of libcgnsio.so (host/cgns_io.c)
+struct CgnsIO {
+    int (*write_grid)(const char *, int, int, int);
+    int (*append_solution)(const char *, int, int, const double *const *, const char *const *);
+    int (*read_field)(const char *, const char *, double *, long long);
+    int (*read_field_int)(const char *, const char *, int *, long long);
+    const char *(*last_error)(void);
+    void (*quiet)(void);
+};
+
+const CgnsIO *cgns_io(std::string *err)
+{
+    static CgnsIO io;
+    static int state = 0;  // 0 not tried, 1 loaded, -1 failed
+    static std::string why;
+    if (state == 0) {
+        char exe[PATH_MAX];
+        const ssize_t len = readlink("/proc/self/exe", exe, sizeof(exe) - 1);
+        std::string dir = ".";
+        if (len > 0) {
+            exe[len] = 0;
+            dir = exe;
+            dir = dir.substr(0, dir.find_last_of('/'));
+        }
+        const std::string path = dir + "/libcgnsio.so";
+        void *h = dlopen(path.c_str(), RTLD_NOW | RTLD_LOCAL);
+        if (!h) {
+            why = "CGNS support unavailable (" + path + ": " + dlerror() + ")";
+            state = -1;
+        } else {
+            io.write_grid = (int (*)(const char *, int, int, int))dlsym(h, "cgio_write_grid");
+            io.append_solution = (int (*)(const char *, int, int, const double *const *, const char *const *))dlsym(
+                h, "cgio_append_solution");
+            io.read_field = (int (*)(const char *, const char *, double *, long long))dlsym(h, "cgio_read_field");
+            io.read_field_int = (int (*)(const char *, const char *, int *, long long))dlsym(h, "cgio_read_field_int");
+            io.last_error = (const char *(*)(void))dlsym(h, "cgio_last_error");
+            io.quiet = (void (*)(void))dlsym(h, "cgio_quiet");
+            if (!io.write_grid || !io.append_solution || !io.read_field || !io.read_field_int || !io.last_error ||
+                !io.quiet) {
+                why = "CGNS support unavailable (" + path + " lacks cgio_* symbols)";
+                state = -1;
+            } else {
+                io.quiet();
+                state = 1;
+            }
+        }
+    }
+    if (state < 0) {
+        if (err) *err = why;
+        return nullptr;
+    }
+    return &io;
+}
+
+}  // namespace
+
+bool FieldSource::open(const std::string &fname, std::string *err)
+{
+    fname_ = fname;
+    cgns_ = is_cgns(fname);
+    if (!cgns_) return file.read(fname, err);
+    if (!cgns_io(err)) return false;
+    file.records.reserve(64);  // get() hands out references into this vector
+    if (FILE *fp = fopen(fname.c_str(), "rb")) {
+        fclose(fp);
+        return true;
+    }
+    if (err) *err = "cannot open " + fname;
+    return false;
+}
+
+const FieldRecord *FieldSource::get(const std::string &label, long long n, uint32_t type, std::string *err)
+{
+    if (const FieldRecord *r = file.find(label)) return r;
+    if (!cgns_) return nullptr;
+    const CgnsIO *io = cgns_io(err);
+    if (!io) return nullptr;
+    FieldRecord rec;
+    rec.name = label;
+    rec.type = type;
+    int rc;
+    if (type == 2) {
+        rec.i32.resize((size_t)n);
+        rc = io->read_field_int(fname_.c_str(), label.c_str(), rec.i32.data(), n);
+    } else {
+        rec.f64.resize((size_t)n);
+        rc = io->read_field(fname_.c_str(), label.c_str(), rec.f64.data(), n);
+    }
+    if (rc != 0) {
+        if (rc < 0 && err) *err = io->last_error();
+        return nullptr;
+    }
+    if (file.records.size() == file.records.capacity()) {
+        if (err) *err = "too many CGNS fields requested";
+        return nullptr;
+    }
+    file.records.push_back(std::move(rec));
+    return &file.records.back();
+}
+
+bool SolutionSink::create(const std::string &fname, int Lx, int Ly, int Lz, std::string *err)
+{
+    fname_ = fname;
+    cgns_ = is_cgns(fname);
+    if (cgns_) {
+        const CgnsIO *io = cgns_io(err);
+        if (!io) return false;
+        if (io->write_grid(fname.c_str(), Lx, Ly, Lz) != 0) {
+            if (err) *err = io->last_error();
+            return false;
+        }
+        return true;
+    }
+    FieldFile out;
+    FieldRecord dims;
+    dims.name = "Dims";
+    dims.type = 2;
+    dims.i32 = {Lx, Ly, Lz};
+    out.records.push_back(dims);
+    return out.write(fname, err);
+}
+
+bool SolutionSink::append(long long iter, const std::vector<FieldRecord> &fields, std::string *err)
+{
+    if (cgns_) {
+        const CgnsIO *io = cgns_io(err);
+        if (!io) return false;
+        std::vector<const double *> data;
+        std::vector<const char *> labels;
+        for (const FieldRecord &r : fields) {
+            data.push_back(r.f64.data());
+            labels.push_back(r.name.c_str());
+        }
+        if (io->append_solution(fname_.c_str(), (int)iter, (int)fields.size(), data.data(), labels.data()) != 0) {
+            if (err) *err = io->last_error();
+            return false;
+        }
+        return true;
+    }
+    std::vector<FieldRecord> recs = fields;
+    for (FieldRecord &r : recs) r.name += "@" + std::to_string(iter);
+    return FieldFile::append(fname_, recs, err);
 }
 
 }  // namespace lbmc_host

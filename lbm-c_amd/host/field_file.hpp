@@ -3,9 +3,11 @@
 // LBM-C reads its per-node inputs from zone 1 / solution 1 of a CGNS file by
 // field name ("Porosity", "ForceX", "MicroBC", ...,
 // LBM-C/cgns/cgns_input_handler.cu:57-103) and appends FlowSolutions to a
-// CGNS output (LBM-C/cgns/cgns_output_handler.cu:264-293).  The image has no
-// CGNS library, so inputs and outputs use this self-describing container
-// with the same field names:
+// CGNS output (LBM-C/cgns/cgns_output_handler.cu:264-293).  File names
+// ending in ".cgns" are read and written as CGNS/HDF5 through libcgnsio.so
+// (host/cgns_io.c, loaded from the executable's directory on first use);
+// every other name uses this self-describing container with the same field
+// names:
 //
 //   "LBMCFLD1"                         8-byte magic
 //   uint32 nrecords
@@ -38,6 +40,35 @@ public:
 
     const FieldRecord *find(const std::string &name) const;
     std::vector<FieldRecord> records;
+};
+
+// true for names ending in ".cgns"
+bool is_cgns(const std::string &fname);
+
+// Named per-node inputs from an LBMCFLD1 file or a CGNS file (zone 1,
+// solution 1).  get() returns nullptr when the field is absent.
+class FieldSource {
+public:
+    bool open(const std::string &fname, std::string *err);
+    const FieldRecord *get(const std::string &label, long long n, uint32_t type, std::string *err);
+    // the fields read so far (all records for an LBMCFLD1 file)
+    FieldFile file;
+
+private:
+    std::string fname_;
+    bool cgns_ = false;
+};
+
+// Time-series solution output: LBMCFLD1 records "<field>@<iter>" after a
+// "Dims" record, or CGNS FlowSolutions "Solution @ Time = <iter>".
+class SolutionSink {
+public:
+    bool create(const std::string &fname, int Lx, int Ly, int Lz, std::string *err);
+    bool append(long long iter, const std::vector<FieldRecord> &fields, std::string *err);
+
+private:
+    std::string fname_;
+    bool cgns_ = false;
 };
 
 }  // namespace lbmc_host

@@ -60,10 +60,15 @@ void ModelBuilder::constant_loader()
 
 namespace {
 
-const FieldRecord &need(const FieldFile &f, const std::string &label, const std::string &fname,
+const FieldRecord &need(FieldSource &f, const std::string &label, const std::string &fname,
                         long long n, uint32_t type, std::ostream &log)
 {
-    const FieldRecord *r = f.find(label);
+    std::string err;
+    const FieldRecord *r = f.get(label, n, type, &err);
+    if (!r && !err.empty()) {
+        fprintf(stderr, "ERROR: Input Handler: %s\n", err.c_str());
+        exit(-1);
+    }
     if (!r) {
         // cgns_input_handler.cu:96-98
         log << "\nInput Handler: " << label << " not found in file \"" << fname << "\"\n";
@@ -84,9 +89,9 @@ const FieldRecord &need(const FieldFile &f, const std::string &label, const std:
 // variable_loader, LBM-C/model_builder.cu:173-231
 void ModelBuilder::variable_loader()
 {
-    FieldFile in;
+    FieldSource in;
     std::string err;
-    if (!in.read(project.domain_fname, &err)) {
+    if (!in.open(project.domain_fname, &err)) {
         fprintf(stderr, "ERROR: Input Handler: %s\n", err.c_str());
         exit(-1);
     }
@@ -129,6 +134,7 @@ void ModelBuilder::variable_loader()
     }
     // other Init values leave the populations as allocated (zero), where the
     // reference leaves them uninitialised (model_builder.cu:227-230)
+    inputs = in.file;
 }
 
 // load_static_IC, LBM-C/model_builder.cu:233-248

@@ -10,6 +10,7 @@
 #include <string>
 
 #include "../../include/lbmc.h"
+#include "field_file.hpp"
 #include "infile_reader.hpp"
 
 namespace lbmc_host {
@@ -31,6 +32,7 @@ public:
     ProjectStrings project;
     long long domain_size = 0;
     long long t_start = 0;  // first time index (a checkpoint restart sets it)
+    FieldFile inputs;       // the per-node input fields that were loaded
     int dim() const { return dc.lattice == 0 ? 2 : 3; }
     int q() const { return dc.lattice == 0 ? 9 : 15; }
 

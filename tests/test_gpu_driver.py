@@ -124,3 +124,35 @@ def test_driver_checkpoint_restart(tmp_path):
     a = fieldfile.read(tmp_path / "full.lbmf")["VelocityX@300"]
     b = fieldfile.read(tmp_path / "b.lbmf")["VelocityX@120"]
     assert np.array_equal(a, b)
+
+
+def test_driver_cgns_in_and_out(tmp_path):
+    """DomainFile and OutputFile as CGNS (through libcgnsio.so): same result
+    as the field-file run and the oracle."""
+    import test_cgns_io as cg
+
+    if not (os.path.exists(cg.LIB) and os.path.exists(cg.H5DUMP)):
+        pytest.skip("libcgnsio.so or h5dump not available")
+    Lx, Ly = 32, 16
+    geo = np.zeros((Ly, Lx))
+    geo[0, :] = 1
+    geo[-1, :] = 1
+    L = cg.lib()
+    fin = str(tmp_path / "in.cgns")
+    assert L.cgio_write_grid(fin.encode(), Lx, Ly, 1) == 0
+    cg.append(L, fin, 0, {"Porosity": geo.reshape(-1), "ForceX": np.full(Lx * Ly, 1e-5),
+                          "ForceY": np.zeros(Lx * Ly)})
+    case = write_case(tmp_path, maxt=300, fileout=150, screen=0, domain="unused.lbmf", out="out.cgns")
+    text = case.read_text().replace(str(tmp_path / "unused.lbmf"), fin)
+    case.write_text(text)
+    r = run(case)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Input Handler: Porosity loaded" in r.stdout
+    fout = str(tmp_path / "out.cgns")
+    shape, ux = cg.dump_data(fout, "/Base/LBM-C Output Data/Solution @ Time = 300/VelocityX/ data")
+    assert shape == (Ly, Lx)
+    o = oracle_channel(Lx, Ly, 300)
+    _, u = o.get_macros()
+    # h5dump prints enough digits to tell FP64 values apart at this size
+    assert np.allclose(ux, u[0], rtol=1e-12, atol=1e-18)
+    assert list(cg.dump_data(fout, "/Base/TimeIterValues/IterationValues/ data")[1]) == [150, 300]
