@@ -38,7 +38,8 @@ def append(L, fn, it, fields):
 
 
 def dump_data(fn, path):
-    out = subprocess.run([H5DUMP, "-w", "0", "-d", path, fn], capture_output=True, text=True, check=True).stdout
+    out = subprocess.run([H5DUMP, "-w", "0", "-m", "%.17g", "-d", path, fn], capture_output=True, text=True,
+                         check=True).stdout
     dims = re.search(r"DATASPACE\s+SIMPLE \{ \( ([0-9, ]+) \)", out).group(1)
     shape = tuple(int(v) for v in dims.split(","))
     body = out[out.index("DATA {") + 6:]
