@@ -160,17 +160,19 @@ def main():
         isd = np.full(7, 1e3, dtype=np.float32)
         isd[6] = 1e2
         model = lbmc.Mlp(closure.init_params(7), mu=mu, isd=isd, out_a=0.005, out_b=0.515)
-        c = make_lattice(lbmc, L, lbmc.FP32, fields)
-        c.set_closure(model)
-        c.time_steps(0, max(5, args.warmup // 5), 0)
-        csteps = max(20, args.steps // 5)
-        cms = c.time_steps(args.warmup, csteps, 0)
-        c.close()
+        closure_line = {"mfma_flop_per_node": 2 * (16 * 64 + 64 * 64 + 64 * 16)}
+        for every in (1, 2, 4):
+            c = make_lattice(lbmc, L, lbmc.FP32, fields)
+            c.set_closure(model)
+            c.set_closure_interval(every)
+            c.time_steps(0, max(8, args.warmup // 5), 0)
+            csteps = max(32, args.steps // 5)
+            cms = c.time_steps(args.warmup, csteps, 0)
+            c.close()
+            closure_line[f"every_{every}"] = {
+                "MLUPS": round(N * csteps / (cms * 1e-3) / 1e6, 1), "ms_per_step": round(cms / csteps, 5),
+                "vs_smagorinsky_step": round((cms / csteps) / ms_step, 3)}
         model.close()
-        closure_line = {"MLUPS": round(N * csteps / (cms * 1e-3) / 1e6, 1),
-                        "ms_per_step": round(cms / csteps, 5),
-                        "vs_smagorinsky_step": round((cms / csteps) / ms_step, 3),
-                        "mfma_flop_per_node": 2 * (16 * 64 + 64 * 64 + 64 * 16)}
         log(f"bench: closure {closure_line}")
 
     ensemble_line = None

@@ -224,6 +224,13 @@ lbmc_status lbmc_mlp_train(lbmc_mlp *m, int epochs, int batch, float lr, unsigne
  * weights, or off with model = null. */
 lbmc_status lbmc_set_closure(lbmc_ctx *ctx, const lbmc_mlp *model, double tau_lo, double tau_hi);
 
+/* Evaluate the network on calls t with t % every == 0 (and on the first call
+ * after any change of state: set_fields, init_*, set_f, set_closure) and
+ * relax the calls in between with each node's recorded tau (4 bytes per
+ * node per call; nodes whose last evaluation fell back use the Smagorinsky
+ * tau of the current state).  every = 1 (the default) evaluates every call. */
+lbmc_status lbmc_set_closure_interval(lbmc_ctx *ctx, int every);
+
 /* ---- Batches of lattices ---------------------------------------------------
  *
  * Steps an ensemble of independent lattices of different sizes together:

@@ -19,7 +19,7 @@
 
 namespace lbmc {
 
-template <class L, class St, bool GUO>
+template <class L, class St, bool GUO, bool RECORD>
 __device__ __forceinline__ void closure_update(const StepArgs<float> &a, int n, bool live, float (&v)[L::Q],
                                                const MlpInfer &w, mlp_v8b *xw)
 {
@@ -40,8 +40,9 @@ __device__ __forceinline__ void closure_update(const StepArgs<float> &a, int n, 
     if (!live) return;
     if (m == 0) {
         float t = w.sc.out_a * y + w.sc.out_b;
-        if (!(t >= w.sc.lo && t <= w.sc.hi))
-            t = a.c_smag > 0.f ? fast_smag_tau<L, St::shifted>(st, a.tau, a.c_smag) : a.tau;
+        const bool in_band = t >= w.sc.lo && t <= w.sc.hi;
+        if constexpr (RECORD) a.tcache[n] = in_band ? t : __builtin_nanf("");
+        if (!in_band) t = a.c_smag > 0.f ? fast_smag_tau<L, St::shifted>(st, a.tau, a.c_smag) : a.tau;
         fast_relax<L, St::shifted, GUO>(v, st, t, a.tau, F);
         if (a.store) {
             sfor<0, D>([&](auto dd) {
@@ -55,7 +56,7 @@ __device__ __forceinline__ void closure_update(const StepArgs<float> &a, int n, 
     }
 }
 
-template <class L, class St, bool GUO, bool WIDE>
+template <class L, class St, bool GUO, bool RECORD, bool WIDE>
 __global__ __launch_bounds__(kBlock, 4) void closure_step_kernel(const StepArgs<float> a)
 {
     __shared__ MlpInfer w;
@@ -76,9 +77,48 @@ __global__ __launch_bounds__(kBlock, 4) void closure_step_kernel(const StepArgs<
         slot_offsets<L>(a, n, dd);
         float v[L::Q];
         gather_node<L, St, WIDE>(a, n, dd, v);
-        closure_update<L, St, GUO>(a, n, live, v, w, xw);
+        closure_update<L, St, GUO, RECORD>(a, n, live, v, w, xw);
         if (live) scatter_node<L, St, WIDE>(a, n, dd, v);
     }
+}
+
+// A call between evaluations: plain fluid nodes relax with the recorded tau
+// (the Smagorinsky tau of the current state where the record is NaN); one
+// thread per node as in step_kernel, no network.  4 more bytes per node.
+template <class L, class St, bool GUO, bool WIDE>
+__global__ __launch_bounds__(kBlock) void closure_reuse_kernel(const StepArgs<float> a)
+{
+    const int n = (int)blockIdx.x * kBlock + (int)threadIdx.x;
+    if (n >= a.nnodes) return;
+    int dd[3][3];
+    slot_offsets<L>(a, n, dd);
+    float v[L::Q];
+    gather_node<L, St, WIDE>(a, n, dd, v);
+    if (a.mask[n] == 0) {
+        constexpr int D = L::D;
+        float F[3] = {0.f, 0.f, 0.f};
+        if constexpr (GUO) {
+            sfor<0, D>([&](auto dd2) {
+                constexpr int d = decltype(dd2)::value;
+                F[d] = a.force[0] ? a.force[d][n] : a.F[d];
+            });
+        }
+        FastState<L> st;
+        fast_prepare<L, St::shifted, GUO>(v, F, st);
+        float t = a.tcache[n];
+        if (t != t) t = a.c_smag > 0.f ? fast_smag_tau<L, St::shifted>(st, a.tau, a.c_smag) : a.tau;
+        fast_relax<L, St::shifted, GUO>(v, st, t, a.tau, F);
+        if (a.store) {
+            sfor<0, D>([&](auto dd2) {
+                constexpr int d = decltype(dd2)::value;
+                a.u[d][n] = st.u[d];
+            });
+            a.rho[n] = st.rho;
+        }
+    } else {
+        node_update<L, St, 0, GUO, true, false>(a, n, v);
+    }
+    scatter_node<L, St, WIDE>(a, n, dd, v);
 }
 
 }  // namespace lbmc

@@ -43,8 +43,14 @@ struct lbmc_ctx {
     uint16_t cls_host[256] = {};
     lbmc::KernelKey key{};
 
-    // learned closure: operand image (device), on when key.closure
+    // learned closure: operand image (device), on when key.closure; with an
+    // evaluation interval K > 1 the network runs on calls t % K == 0 (and the
+    // first call after any change of state) and the other calls reuse the
+    // recorded per-node tau
     void *mlp_img = nullptr;
+    int closure_every = 1;
+    float *tcache = nullptr;
+    bool tcache_valid = false;
 
     // launch shape
     int grid = 1, remap = 0;
@@ -109,6 +115,8 @@ inline void fill_args(const lbmc_ctx *c, StepArgs<C> &a, long long t, int store)
     a.porous = c->porous ? 1 : 0;
     a.quirks = c->desc.quirks;
     a.mlp = c->mlp_img;
+    a.tcache = c->tcache;
+    a.tmode = 0;
 }
 
 }  // namespace lbmc

@@ -114,7 +114,14 @@ lbmc_status launch_step(lbmc_ctx *c, long long t, int store)
             }
             if (e != hipSuccess) return hip_fail(c, e, "sf gather launch");
         }
-        const int grid = c->key.closure ? c->closure_grid : c->grid;
+        int grid = c->grid;
+        if (c->key.closure) {
+            if (c->closure_every > 1) {
+                a.tmode = (!c->tcache_valid || t % c->closure_every == 0) ? 1 : 2;
+                if (a.tmode == 1) c->tcache_valid = true;
+            }
+            if (a.tmode != 2) grid = c->closure_grid;
+        }
         switch (c->storage) {
         case LBMC_FP32: e = launch_step_fp32(c->key, a, grid, c->stream); break;
         case LBMC_FP16S: e = launch_step_fp16s(c->key, a, grid, c->stream); break;
@@ -153,6 +160,15 @@ void drop_graph(lbmc_ctx *c)
     c->graph_phase = -1;
 }
 
+// the closure's recorded tau belongs to the state it was computed from
+void invalidate_tau_record(lbmc_ctx *c)
+{
+    if (c->tcache_valid) {
+        c->tcache_valid = false;
+        drop_graph(c);
+    }
+}
+
 long long lcm_ll(long long a, long long b)
 {
     long long x = a, y = b;
@@ -169,8 +185,16 @@ long long lcm_ll(long long a, long long b)
 // calls therefore repeats exactly: capture it once into a graph and replay.
 lbmc_status run_steps(lbmc_ctx *c, long long t0, int nsteps, int store_every)
 {
+    if (c->key.closure && c->closure_every > 1 && !c->tcache_valid && nsteps > 0) {
+        // the first call after a change of state evaluates the network; the
+        // captured calls then start from a valid record
+        if (lbmc_status st = launch_step(c, t0, store_every > 0 && ((t0 + 1) % store_every) == 0)) return st;
+        t0++;
+        nsteps--;
+    }
     long long period = lcm_ll(lcm_ll(c->L[0], c->L[1]), c->L[2]);
     if (store_every > 0) period = lcm_ll(period, store_every);
+    if (c->key.closure && c->closure_every > 1) period = lcm_ll(period, c->closure_every);
     const bool graphable = c->use_graph && period <= 4096 && nsteps >= 2 * period;
     long long t = t0;
     long long left = nsteps;
@@ -327,6 +351,7 @@ lbmc_status lbmc_destroy(lbmc_ctx *c)
     p = c->partials; free_dev(p);
     p = c->scalar; free_dev(p);
     free_dev(c->mlp_img);
+    p = c->tcache; free_dev(p); c->tcache = nullptr;
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -469,6 +494,7 @@ lbmc_status lbmc_set_fields(lbmc_ctx *c, const double *geometry, const int *macr
         HIP_TRY(c, hipMemcpy(c->sfidx, idx.data(), sizeof(int) * (size_t)N, hipMemcpyHostToDevice), "sf upload");
     }
     HIP_TRY(c, hipDeviceSynchronize(), "set_fields sync");
+    c->tcache_valid = false;
     drop_graph(c);
     c->fields_set = true;
     return LBMC_OK;
@@ -502,6 +528,7 @@ lbmc_status lbmc_get_links(lbmc_ctx *c, uint16_t *links)
 lbmc_status lbmc_init_static(lbmc_ctx *c)
 {
     if (!c) return fail(nullptr, LBMC_ERR_INVALID, "null context");
+    invalidate_tau_record(c);
     HIP_TRY(c, hipSetDevice(c->desc.device), "hipSetDevice");
     HIP_TRY(c, launch_init_static(c->desc.lattice, c->storage, c->f, c->npad, c->N, c->stream), "init");
     HIP_TRY(c, hipStreamSynchronize(c->stream), "init sync");
@@ -511,6 +538,7 @@ lbmc_status lbmc_init_static(lbmc_ctx *c)
 lbmc_status lbmc_init_equilibrium(lbmc_ctx *c, const double *rho, const double *const *u)
 {
     if (!c || !rho || !u) return fail(c, LBMC_ERR_INVALID, "null argument");
+    invalidate_tau_record(c);
     for (int d = 0; d < c->D; d++)
         if (!u[d]) return fail(c, LBMC_ERR_INVALID, "u[%d] is null", d);
     HIP_TRY(c, hipSetDevice(c->desc.device), "hipSetDevice");
@@ -630,6 +658,7 @@ lbmc_status lbmc_get_f(lbmc_ctx *c, double *const *f)
 lbmc_status lbmc_set_f(lbmc_ctx *c, const double *const *f)
 {
     if (!c || !f) return fail(c, LBMC_ERR_INVALID, "null argument");
+    invalidate_tau_record(c);
     HIP_TRY(c, hipSetDevice(c->desc.device), "hipSetDevice");
     double *tmp = nullptr;
     HIP_TRY(c, hipMalloc((void **)&tmp, sizeof(double) * (size_t)c->N), "f write buffer");
@@ -680,6 +709,21 @@ lbmc_status lbmc_set_closure(lbmc_ctx *c, const lbmc_mlp *model, double tau_lo, 
     if (st != LBMC_OK) return fail(c, st, "lbmc_set_closure: model is on another device or the image build failed");
     HIP_TRY(c, hipStreamSynchronize(c->stream), "closure image build");
     c->key.closure = 1;
+    c->tcache_valid = false;
+    drop_graph(c);
+    return LBMC_OK;
+}
+
+lbmc_status lbmc_set_closure_interval(lbmc_ctx *c, int every)
+{
+    if (!c) return fail(nullptr, LBMC_ERR_INVALID, "lbmc_set_closure_interval: null context");
+    if (every < 1 || every > 4096) return fail(c, LBMC_ERR_INVALID, "lbmc_set_closure_interval: interval %d not in [1, 4096]", every);
+    if (every > 1 && !c->tcache) {
+        HIP_TRY(c, hipSetDevice(c->desc.device), "hipSetDevice");
+        HIP_TRY(c, hipMalloc((void **)&c->tcache, sizeof(float) * (size_t)c->npad), "closure tau record");
+    }
+    c->closure_every = every;
+    c->tcache_valid = false;
     drop_graph(c);
     return LBMC_OK;
 }

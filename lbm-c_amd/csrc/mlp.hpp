@@ -78,9 +78,13 @@ constexpr float GELU_C5 = 0.007179733831435442f;
 constexpr float GELU_C7 = -0.0004616279620677233f;
 constexpr float GELU_C9 = 1.2130119102948811e-05f;
 
+// x clamped to [-R, R] by one v_med3 (fminf/fmaxf would add a NaN-quieting
+// v_max; a NaN x still yields a NaN GELU through the final x * Phi)
+__device__ __forceinline__ float gelu_clamp(float x) { return __builtin_amdgcn_fmed3f(x, -GELU_R, GELU_R); }
+
 __device__ __forceinline__ float gelu_fwd(float x)
 {
-    const float xc = fminf(fmaxf(x, -GELU_R), GELU_R);
+    const float xc = gelu_clamp(x);
     const float t = xc * xc;
     float p = fmaf(t, GELU_C9, GELU_C7);
     p = fmaf(t, p, GELU_C5);
@@ -93,7 +97,7 @@ __device__ __forceinline__ float gelu_fwd(float x)
 // d GELU / dx = Phi(x) + x Phi'(x), Phi' zero where x or Phi is clamped
 __device__ __forceinline__ float gelu_grad(float x)
 {
-    const float xc = fminf(fmaxf(x, -GELU_R), GELU_R);
+    const float xc = gelu_clamp(x);
     const float t = xc * xc;
     float p = fmaf(t, GELU_C9, GELU_C7);
     p = fmaf(t, p, GELU_C5);

@@ -70,6 +70,8 @@ struct StepArgs {
     int porous;              // NTPOR/MRTPOR with porous nodes present
     unsigned quirks;
     const void *mlp;         // closure weights (MlpImage) when the closure is on
+    float *tcache;           // closure: per-node tau of the last evaluation (NaN: fall back)
+    int tmode;               // closure: 0 evaluate, 1 evaluate and record, 2 use the record
 };
 
 }  // namespace lbmc

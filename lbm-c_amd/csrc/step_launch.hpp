@@ -21,14 +21,25 @@ static hipError_t launch_one(const StepArgs<typename St::C> &a, int grid, hipStr
     return hipGetLastError();
 }
 
+// closure calls: tmode 0 evaluate, 1 evaluate and record tau, 2 reuse the
+// record (grid: the persistent closure grid for 0/1, one block per 256 nodes
+// for 2)
+template <class L, class St, bool GUO, bool W>
+static hipError_t launch_closure_w(const StepArgs<float> &a, int grid, hipStream_t s)
+{
+    if (a.tmode == 2)
+        hipLaunchKernelGGL((closure_reuse_kernel<L, St, GUO, W>), dim3(grid), dim3(kBlock), 0, s, a);
+    else if (a.tmode == 1)
+        hipLaunchKernelGGL((closure_step_kernel<L, St, GUO, true, W>), dim3(grid), dim3(kBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL((closure_step_kernel<L, St, GUO, false, W>), dim3(grid), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
 template <class L, class St, bool GUO>
 static hipError_t launch_closure(const StepArgs<float> &a, int grid, hipStream_t s, bool wide)
 {
-    if (wide)
-        hipLaunchKernelGGL((closure_step_kernel<L, St, GUO, true>), dim3(grid), dim3(kBlock), 0, s, a);
-    else
-        hipLaunchKernelGGL((closure_step_kernel<L, St, GUO, false>), dim3(grid), dim3(kBlock), 0, s, a);
-    return hipGetLastError();
+    return wide ? launch_closure_w<L, St, GUO, true>(a, grid, s) : launch_closure_w<L, St, GUO, false>(a, grid, s);
 }
 
 template <class L, class St>

@@ -38,7 +38,7 @@ EXPORTS = [
     "lbmc_kinetic_energy", "lbmc_sync", "lbmc_info", "lbmc_copy_bandwidth",
     "lbmc_mlp_create", "lbmc_mlp_destroy", "lbmc_mlp_last_error", "lbmc_mlp_set_params",
     "lbmc_mlp_get_params", "lbmc_mlp_set_scaling", "lbmc_mlp_forward", "lbmc_mlp_loss_grad",
-    "lbmc_mlp_set_data", "lbmc_mlp_train", "lbmc_set_closure",
+    "lbmc_mlp_set_data", "lbmc_mlp_train", "lbmc_set_closure", "lbmc_set_closure_interval",
     "lbmc_batch_create", "lbmc_batch_destroy", "lbmc_batch_last_error", "lbmc_batch_run",
     "lbmc_batch_time_steps", "lbmc_batch_autotune", "lbmc_batch_set_grid", "lbmc_batch_info",
 ]
@@ -120,6 +120,7 @@ def _load():
         "lbmc_mlp_train": (ct.c_int, [P, ct.c_int, ct.c_int, ct.c_float, ct.c_ulonglong,
                                       ct.POINTER(ct.c_double)]),
         "lbmc_set_closure": (ct.c_int, [P, P, ct.c_double, ct.c_double]),
+        "lbmc_set_closure_interval": (ct.c_int, [P, ct.c_int]),
         "lbmc_batch_create": (ct.c_int, [ct.POINTER(P), ct.c_int, ct.POINTER(P)]),
         "lbmc_batch_destroy": (ct.c_int, [P]),
         "lbmc_batch_last_error": (ct.c_char_p, [P]),
@@ -325,6 +326,11 @@ class Lattice:
         self._closure_model = model
         h = model._h if model is not None else None
         self._check(_lib.lbmc_set_closure(self._h, h, float(tau_lo), float(tau_hi)))
+
+    def set_closure_interval(self, every):
+        """Evaluate the closure network every `every` calls (reusing each
+        node's recorded tau in between)."""
+        self._check(_lib.lbmc_set_closure_interval(self._h, int(every)))
 
 
 def copy_bandwidth(nbytes, reps=20, device=0):

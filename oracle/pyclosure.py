@@ -8,8 +8,9 @@ optional uniform Guo force, the reference's call semantics
 (LBM-C/solver.cu:10-136): call 0 collides in place, every later call first
 pulls f_i(x) <- f_i(x - e_i) and then collides; solids bounce back
 (LBM-C/collision.cu:423-443).  At fluid nodes tau comes from the network
-(oracle/pymlp.py, bf16 quantisation points emulated) as out_a * y + out_b,
-or from the Smagorinsky formula (LBM-C/collision.cu:445-463, restated in
+(oracle/pymlp.py, bf16 quantisation points emulated) as out_a * y + out_b
+(evaluated on calls t % every == 0 and the first call, the recorded tau
+reused in between), or from the Smagorinsky formula (LBM-C/collision.cu:445-463, restated in
 closed form: tau' = (tau + sqrt(tau^2 + 36 C^2 (sum|Pi_aa| + 2 sum_{a<b}
 |Pi_ab|) / rho)) / 2) when that leaves [lo, hi].  The Guo terms follow
 LBM-C/collision.cu:69-111 with the base tau in the source.  Lattice tables
@@ -26,7 +27,7 @@ import pyoracle
 
 class ClosureOracle:
     def __init__(self, lattice, length, tau, c_smag, params, mu, isd, out_a, out_b, lo, hi,
-                 force=None, solid=None, quant=True):
+                 force=None, solid=None, quant=True, every=1):
         self.lattice = lattice
         self.dim = 2 if lattice == pyoracle.D2Q9 else 3
         e, w, opp, _, _ = pyoracle.tables(lattice)
@@ -46,6 +47,8 @@ class ClosureOracle:
         n = int(np.prod(self.shape))
         self.solid = np.zeros(n, dtype=bool) if solid is None else np.asarray(solid, dtype=bool).reshape(-1)
         self.quant = quant
+        self.every = int(every)
+        self.record = None   # tau of the last evaluation per node, NaN where it fell back
         self.f = None
         self.calls = 0
         self.fallback = None
@@ -97,10 +100,17 @@ class ClosureOracle:
     def collide(self):
         f = self.f
         rho, u, feq, pi, x = self.features(f)
-        xn = ((x.astype(np.float32) - self.mu) * self.isd).astype(np.float64)
-        y, _ = pymlp.forward(self.p, xn, quant=self.quant)
-        t = self.out_a * y + self.out_b
-        bad = ~((t >= self.lo) & (t <= self.hi))
+        if self.every <= 1 or self.record is None or self.calls % self.every == 0:
+            xn = ((x.astype(np.float32) - self.mu) * self.isd).astype(np.float64)
+            y, _ = pymlp.forward(self.p, xn, quant=self.quant)
+            t = self.out_a * y + self.out_b
+            bad = ~((t >= self.lo) & (t <= self.hi))
+            self.record = np.where(bad, np.nan, t)
+        else:
+            # between evaluations: the recorded tau, Smagorinsky of the current
+            # state where the record fell back
+            t = np.nan_to_num(self.record, nan=0.0)
+            bad = np.isnan(self.record)
         if self.c_smag > 0:
             t = np.where(bad, self.smagorinsky(rho, pi), t)
         else:

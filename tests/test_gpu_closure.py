@@ -113,7 +113,7 @@ def test_training_fits_and_is_deterministic():
     assert np.mean((y - t) ** 2) < 0.1 * np.var(t)
 
 
-def _closure_case(lattice, L, tau, c_smag, storage, force=None, solid=None, seed=20, steps=12):
+def _closure_case(lattice, L, tau, c_smag, storage, force=None, solid=None, seed=20, steps=12, every=1):
     """Random network whose tau' falls inside the band at most nodes and
     outside it at some; returns (device macros, oracle macros, fallback share)."""
     dim = 2 if lattice == lbmc.D2Q9 else 3
@@ -143,10 +143,12 @@ def _closure_case(lattice, L, tau, c_smag, storage, force=None, solid=None, seed
     lat.init_equilibrium(rho, u)
     m = lbmc.Mlp(pymlp.flatten(p), mu=mu, isd=isd, out_a=out_a, out_b=out_b)
     lat.set_closure(m, lo, hi)
+    if every > 1:
+        lat.set_closure_interval(every)
     lat.run(0, steps, 1)
     rd, ud = lat.get_macros()
     co = pyclosure.ClosureOracle(lattice, L, tau, c_smag, p, mu, isd, out_a, out_b, lo, hi,
-                                 force=force, solid=solid)
+                                 force=force, solid=solid, every=every)
     co.init_equilibrium(rho, u)
     ro, uo = co.run(steps)
     lat.close()
@@ -175,6 +177,17 @@ def test_closure_step_d2q9_guo_solids_matches_oracle():
     assert 0.02 < fb < 0.98, fb
     assert np.max(np.abs(ud - uo)) < 2e-6
     assert np.max(np.abs(rd - ro)) < 2e-6
+
+
+@pytest.mark.parametrize("every", [2, 3])
+def test_closure_interval_matches_oracle(every):
+    # 16^3 (lcm of the lengths with the interval is small): 40 calls go
+    # through captured graphs after the first evaluating call
+    print(f"[closure] interval {every}", flush=True)
+    (rd, ud), (ro, uo), fb = _closure_case(lbmc.D3Q15, (16, 16, 16), 0.51, 0.17, lbmc.FP32, steps=40,
+                                           every=every)
+    assert np.max(np.abs(ud - uo)) < 4e-6
+    assert np.max(np.abs(rd - ro)) < 4e-6
 
 
 def test_closure_off_restores_smagorinsky_bitwise():
