@@ -41,12 +41,15 @@ struct lbmc_batch {
     void *dmembers = nullptr;  // BatchMember<C>[count]
     int2 *chunks = nullptr;
     int nchunks = 0;
+    const void *mlp = nullptr;  // closure weights shared by the closure members (a member's image)
     long long nodes = 0;
     int grid = 1;
     long long t_next = -1;  // the call the device arguments describe
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int graph_steps = 0;
+    const lbmc_mlp *closure_model = nullptr;
+    double closure_lo = 0, closure_hi = 0;
     std::string err;
 };
 
@@ -75,6 +78,7 @@ lbmc_status bfail(lbmc_batch *b, lbmc_status st, const char *fmt, ...)
 int member_variant(const lbmc_ctx *c)
 {
     const KernelKey &k = c->key;
+    if (k.closure) return 8 + (k.guo ? 1 : 0);
     const int bit0 = k.family == 0 ? (k.smag ? 1 : 0) : (k.mfix ? 1 : 0);
     return (k.family << 2) | ((k.guo ? 1 : 0) << 1) | bit0;
 }
@@ -87,8 +91,12 @@ lbmc_status upload_members(lbmc_batch *b, long long t0, int store_every)
     std::vector<BatchMember<C>> hm((size_t)b->count);
     for (int k = 0; k < b->count; k++) {
         const lbmc_ctx *c = b->members[(size_t)k];
-        if (c->nsf || c->key.closure || c->key.wide || !c->fields_set)
-            return bfail(b, LBMC_ERR_UNSUPPORTED, "batch member %d: sf nodes, closure or state not supported", k);
+        if (c->nsf || c->key.wide || !c->fields_set)
+            return bfail(b, LBMC_ERR_UNSUPPORTED, "batch member %d: sf nodes or state not supported", k);
+        if (c->key.closure && (c->closure_model != b->closure_model || c->closure_lo != b->closure_lo ||
+                               c->closure_hi != b->closure_hi || c->closure_every != 1))
+            return bfail(b, LBMC_ERR_UNSUPPORTED,
+                         "batch member %d: closure members must share one model and band, interval 1", k);
         BatchMember<C> &m = hm[(size_t)k];
         memset(&m, 0, sizeof(m));
         const int se = store_every > 0 ? store_every : 0;
@@ -109,9 +117,9 @@ lbmc_status launch_one_step(lbmc_batch *b)
     hipError_t e;
     switch (b->storage) {
     case LBMC_FP64: e = launch_batch_fp64(b->lattice, b->dmembers, b->chunks, b->nchunks, b->grid, b->stream); break;
-    case LBMC_FP32: e = launch_batch_st<StFP32>(b->lattice, b->dmembers, b->chunks, b->nchunks, b->grid, b->stream); break;
-    case LBMC_FP16S: e = launch_batch_st<StFP16S>(b->lattice, b->dmembers, b->chunks, b->nchunks, b->grid, b->stream); break;
-    default: e = launch_batch_st<StBF16S>(b->lattice, b->dmembers, b->chunks, b->nchunks, b->grid, b->stream); break;
+    case LBMC_FP32: e = launch_batch_st<StFP32>(b->lattice, b->dmembers, b->chunks, b->nchunks, b->grid, b->mlp, b->stream); break;
+    case LBMC_FP16S: e = launch_batch_st<StFP16S>(b->lattice, b->dmembers, b->chunks, b->nchunks, b->grid, b->mlp, b->stream); break;
+    default: e = launch_batch_st<StBF16S>(b->lattice, b->dmembers, b->chunks, b->nchunks, b->grid, b->mlp, b->stream); break;
     }
     if (e != hipSuccess) return bfail(b, LBMC_ERR_DEVICE, "batch step launch: %s", hipGetErrorString(e));
     const int nb = (b->count + 63) / 64;
@@ -194,6 +202,9 @@ lbmc_status lbmc_batch_create(lbmc_ctx *const *members, int count, lbmc_batch **
     const lbmc_ctx *c0 = members[0];
     if (!c0) return bfail(nullptr, LBMC_ERR_INVALID, "lbmc_batch_create: member 0 is null");
     long long nchunks = 0, nodes = 0;
+    const lbmc_mlp *closure_model = nullptr;
+    double closure_lo = 0, closure_hi = 0;
+    const void *closure_img = nullptr;
     for (int k = 0; k < count; k++) {
         const lbmc_ctx *c = members[k];
         if (!c) return bfail(nullptr, LBMC_ERR_INVALID, "lbmc_batch_create: member %d is null", k);
@@ -202,8 +213,18 @@ lbmc_status lbmc_batch_create(lbmc_ctx *const *members, int count, lbmc_batch **
             return bfail(nullptr, LBMC_ERR_INVALID,
                          "lbmc_batch_create: member %d differs in lattice, storage or device from member 0", k);
         if (c->nsf) return bfail(nullptr, LBMC_ERR_UNSUPPORTED, "lbmc_batch_create: member %d has sf boundary nodes", k);
-        if (c->key.closure)
-            return bfail(nullptr, LBMC_ERR_UNSUPPORTED, "lbmc_batch_create: member %d uses the learned closure", k);
+        if (c->key.closure) {
+            if (c->desc.lattice != c0->desc.lattice || c->closure_every != 1)
+                return bfail(nullptr, LBMC_ERR_UNSUPPORTED, "lbmc_batch_create: member %d: closure with interval > 1", k);
+            if (closure_model && (c->closure_model != closure_model || c->closure_lo != closure_lo ||
+                                  c->closure_hi != closure_hi))
+                return bfail(nullptr, LBMC_ERR_UNSUPPORTED,
+                             "lbmc_batch_create: closure members must share one model and band (member %d)", k);
+            closure_model = c->closure_model;
+            closure_lo = c->closure_lo;
+            closure_hi = c->closure_hi;
+            closure_img = c->mlp_img;
+        }
         if (c->key.wide) return bfail(nullptr, LBMC_ERR_UNSUPPORTED, "lbmc_batch_create: member %d is too large", k);
         for (int j = 0; j < k; j++)
             if (members[j] == c) return bfail(nullptr, LBMC_ERR_INVALID, "lbmc_batch_create: member %d repeats %d", k, j);
@@ -220,6 +241,10 @@ lbmc_status lbmc_batch_create(lbmc_ctx *const *members, int count, lbmc_batch **
     b->members.assign(members, members + count);
     b->nchunks = (int)nchunks;
     b->nodes = nodes;
+    b->closure_model = closure_model;
+    b->closure_lo = closure_lo;
+    b->closure_hi = closure_hi;
+    b->mlp = closure_img;
     auto bail = [&](lbmc_status st) {
         g_batch_error = b->err;
         lbmc_batch_destroy(b);

@@ -8,7 +8,7 @@ namespace lbmc {
 hipError_t launch_batch_fp64(int lattice, const void *members, const int2 *chunks, int nchunks, int grid,
                              hipStream_t s)
 {
-    return launch_batch_st<StFP64>(lattice, members, chunks, nchunks, grid, s);
+    return launch_batch_st<StFP64>(lattice, members, chunks, nchunks, grid, nullptr, s);
 }
 
 }  // namespace lbmc

@@ -5,13 +5,17 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
+#include "closure_kernel.hpp"
 #include "step_kernel.hpp"
 #include "storage.hpp"
 
 namespace lbmc {
 
 // collision variant of a member: bit 2 MRT family, bit 1 Guo forcing, bit 0
-// Smagorinsky (BGK family) or corrected D3Q15 M^-1 (MRT family)
+// Smagorinsky (BGK family) or corrected D3Q15 M^-1 (MRT family); 8 and 9 are
+// BGK-family members with the learned closure (9: with Guo forcing)
 template <class C>
 struct BatchMember {
     StepArgs<C> a;
@@ -41,20 +45,47 @@ __device__ __forceinline__ void batch_node(const StepArgs<typename St::C> &a, in
     }
 }
 
-template <class L, class St>
+// CLOSURE: some members use the learned closure (one shared network, staged
+// into LDS once per block); their chunks run the network over the whole
+// wave, lanes past the member's last node evaluating a copy of it.
+template <class L, class St, bool CLOSURE>
 __global__ __launch_bounds__(kBlock) void batch_step_kernel(const BatchMember<typename St::C> *members,
-                                                            const int2 *chunks, int nchunks)
+                                                            const int2 *chunks, int nchunks, const void *mlp)
 {
+    using C = typename St::C;
+    constexpr bool CL = CLOSURE && std::is_same<C, float>::value;
+    __shared__ typename std::conditional<CL, MlpInfer, char>::type w;
+    __shared__ mlp_v8b xb[CL ? kBlock : 1];
+    if constexpr (CL) {
+        const uint4 *src = (const uint4 *)mlp;
+        uint4 *dst = (uint4 *)&w;
+        for (int k = (int)threadIdx.x; k < (int)(sizeof(MlpInfer) / 16); k += kBlock) dst[k] = src[k];
+        __syncthreads();
+    }
     for (int c = (int)blockIdx.x; c < nchunks; c += (int)gridDim.x) {
         const int2 ch = chunks[c];  // (member, first node)
-        const BatchMember<typename St::C> &m = members[ch.x];
-        const StepArgs<typename St::C> &a = m.a;
+        const BatchMember<C> &m = members[ch.x];
+        const StepArgs<C> &a = m.a;
         const int variant = m.variant;
-        const int n = ch.y + (int)threadIdx.x;
-        if (n < a.nnodes) {
+        const int n0 = ch.y + (int)threadIdx.x;
+        if constexpr (CL) {
+            if (variant >= 8) {
+                const bool live = n0 < a.nnodes;
+                const int n = live ? n0 : a.nnodes - 1;
+                int dd[3][3];
+                slot_offsets<L>(a, n, dd);
+                float v[L::Q];
+                gather_node<L, St, false>(a, n, dd, v);
+                if (variant == 9) closure_update<L, St, true, false>(a, n, live, v, w, xb + (threadIdx.x & ~63u));
+                else closure_update<L, St, false, false>(a, n, live, v, w, xb + (threadIdx.x & ~63u));
+                if (live) scatter_node<L, St, false>(a, n, dd, v);
+                continue;
+            }
+        }
+        if (n0 < a.nnodes) {
             int dd[3][3];
-            slot_offsets<L>(a, n, dd);
-            batch_node<L, St>(a, variant, n, dd);
+            slot_offsets<L>(a, n0, dd);
+            batch_node<L, St>(a, variant, n0, dd);
         }
     }
 }
@@ -88,19 +119,25 @@ __global__ void batch_advance_kernel(BatchMember<C> *members, int count)
 }
 
 template <class L, class St>
-hipError_t launch_batch_lattice(const void *members, const int2 *chunks, int nchunks, int grid, hipStream_t s)
+hipError_t launch_batch_lattice(const void *members, const int2 *chunks, int nchunks, int grid, const void *mlp,
+                                hipStream_t s)
 {
-    hipLaunchKernelGGL((batch_step_kernel<L, St>), dim3(grid), dim3(kBlock), 0, s,
-                       (const BatchMember<typename St::C> *)members, chunks, nchunks);
+    const auto *mm = (const BatchMember<typename St::C> *)members;
+    if (mlp)
+        hipLaunchKernelGGL((batch_step_kernel<L, St, true>), dim3(grid), dim3(kBlock), 0, s, mm, chunks, nchunks, mlp);
+    else
+        hipLaunchKernelGGL((batch_step_kernel<L, St, false>), dim3(grid), dim3(kBlock), 0, s, mm, chunks, nchunks,
+                           mlp);
     return hipGetLastError();
 }
 
+// mlp: the shared closure weights when some member uses the closure, else null
 template <class St>
 hipError_t launch_batch_st(int lattice, const void *members, const int2 *chunks, int nchunks, int grid,
-                           hipStream_t s)
+                           const void *mlp, hipStream_t s)
 {
-    return lattice == 0 ? launch_batch_lattice<D2Q9, St>(members, chunks, nchunks, grid, s)
-                                : launch_batch_lattice<D3Q15, St>(members, chunks, nchunks, grid, s);
+    return lattice == 0 ? launch_batch_lattice<D2Q9, St>(members, chunks, nchunks, grid, mlp, s)
+                        : launch_batch_lattice<D3Q15, St>(members, chunks, nchunks, grid, mlp, s);
 }
 
 }  // namespace lbmc

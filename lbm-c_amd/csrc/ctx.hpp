@@ -48,6 +48,8 @@ struct lbmc_ctx {
     // first call after any change of state) and the other calls reuse the
     // recorded per-node tau
     void *mlp_img = nullptr;
+    const lbmc_mlp *closure_model = nullptr;  // the model the image was built from
+    double closure_lo = 0, closure_hi = 0;
     int closure_every = 1;
     float *tcache = nullptr;
     bool tcache_valid = false;

@@ -116,3 +116,56 @@ def test_batch_rejects_mixed_members():
     assert e.value.status == 4
     for x in (a, b, c):
         x.close()
+
+
+def test_batch_with_closure_members_matches_single_lattices():
+    """D2Q9 members with the learned closure (one shared network; with and
+    without Guo forcing) beside an MRT member: each ends where it ends when
+    stepped on its own."""
+    import pymlp
+    import workloads
+
+    print("[batch] closure members", flush=True)
+    rng = np.random.default_rng(40)
+    p = pymlp.init_params(40)
+    p["b3"] = 0.1
+    mu = np.zeros(7, dtype=np.float32)
+    mu[6] = 1.0
+    isd = np.full(7, 300.0, dtype=np.float32)
+    isd[6] = 100.0
+    model = lbmc.Mlp(pymlp.flatten(p), mu=mu, isd=isd, out_a=0.02, out_b=0.62)
+    specs = []
+    for k, (Lx, Ly) in enumerate([(40, 24), (72, 33), (17, 50)]):
+        geo = workloads.channel(Lx, Ly)
+        geo[Ly // 2, Lx // 3] = 1.0
+        n = Lx * Ly
+        u = [0.02 * rng.standard_normal(n), 0.02 * rng.standard_normal(n)]
+        specs.append(dict(L=(Lx, Ly), geo=geo.reshape(-1), u=u, closure=k != 2, guo=k == 1, mrt=k == 2))
+
+    def build(spec):
+        lat = lbmc.Lattice(lbmc.D2Q9, spec["L"], lbmc.MRT if spec["mrt"] else lbmc.BGK,
+                           forcing=spec["guo"] or spec["mrt"], tau=0.6, tau_mrt=cases.d2q9_rates(0.6),
+                           c_smag=0.1, storage=lbmc.FP32)
+        n = spec["L"][0] * spec["L"][1]
+        force = [np.full(n, 1e-5), np.zeros(n)] if (spec["guo"] or spec["mrt"]) else None
+        lat.set_fields(geometry=spec["geo"], force=force)
+        lat.init_equilibrium(np.ones(n), spec["u"])
+        if spec["closure"]:
+            lat.set_closure(model, 0.6, 0.7)
+        return lat
+
+    steps = 50
+    lats = [build(s) for s in specs]
+    b = lbmc.Batch(lats)
+    b.run(0, steps, 1)
+    got = [lat.get_f() for lat in lats]
+    b.close()
+    for lat in lats:
+        lat.close()
+    for spec, fg in zip(specs, got):
+        lat = build(spec)
+        lat.run(0, steps, 1)
+        fs = lat.get_f()
+        lat.close()
+        assert np.max(np.abs(fg - fs)) <= 1e-6, spec["L"]
+    model.close()
