@@ -12,7 +12,9 @@
 //    equilibrium's second moment is exactly rho (u u + I/3); the norm
 //    sum_ab sqrt(2 Pi_ab^2) = sqrt(2) (sum |Pi_aa| + 2 sum_{a<b} |Pi_ab|);
 //  * shifted storage (g = f - w): the update runs on g directly,
-//    g' = g + omega (geq - g) with geq = feq - w, so no +-w conversions;
+//    g' = g + omega (geq - g) with geq = feq - w, so no +-w conversions, and
+//    geq is formed from s = rho - 1 (rho X - 1 = s X + (X - 1)) so that the
+//    rounding of rho does not enter the non-equilibrium part;
 //  * opposite directions share the even part of feq and of the Guo source.
 #pragma once
 
@@ -85,6 +87,7 @@ struct FastMoments<D2Q9> {
 // g = f - w when SH); u carries the Guo half-force shift (collision.cu:74-78).
 template <class L>
 struct FastState {
+    float s;  // sum of the stored values: rho - 1 when shifted, rho otherwise
     float rho, ir;
     float u[3];
     float P[6];  // second moment of the stored values: xx, yy, zz, xy, yz, xz
@@ -97,6 +100,7 @@ __device__ __forceinline__ void fast_prepare(const float *v, const float *F, Fas
     constexpr float sh = SH ? 1.f : 0.f;
     float s, j[3];
     FastMoments<L>::run(v, s, j, st.P);
+    st.s = s;
     st.rho = sh + s;
     st.ir = recip(st.rho);
     st.u[0] = j[0] * st.ir;
@@ -120,7 +124,7 @@ __device__ __forceinline__ float fast_smag_tau(const FastState<L> &st, float tau
     const float rho = st.rho;
     const float *u = st.u;
     const float *P = st.P;
-    const float dg = (rho - sh) * (1.f / 3.f);
+    const float dg = st.s * (1.f / 3.f);  // (rho - sh) / 3 without the rounding of rho
     float sd = fabsf(P[0] - rho * u[0] * u[0] - dg) + fabsf(P[1] - rho * u[1] * u[1] - dg);
     float so = fabsf(P[3] - rho * u[0] * u[1]);
     if constexpr (L::D > 2) {
@@ -141,7 +145,7 @@ __device__ __forceinline__ void fast_closure_features(const FastState<L> &st, fl
     const float rho = st.rho;
     const float *u = st.u;
     const float *P = st.P;
-    const float dg = (rho - sh) * (1.f / 3.f);
+    const float dg = st.s * (1.f / 3.f);
     x[0] = (P[0] - rho * u[0] * u[0] - dg) * st.ir;
     x[1] = (P[1] - rho * u[1] * u[1] - dg) * st.ir;
     x[3] = (P[3] - rho * u[0] * u[1]) * st.ir;
@@ -177,9 +181,11 @@ __device__ __forceinline__ void fast_relax(float *v, const FastState<L> &st, flo
         uF = u[0] * F[0] + u[1] * F[1];
         if constexpr (D > 2) uF += u[2] * F[2];
     }
-    // rest population
+    // rest population; rho X - sh = s X + sh (X - 1) keeps the shifted
+    // equilibrium free of the rounding of rho = 1 + s
+    const float s = st.s;
     {
-        const float veq = float(L::w[0]) * (rho * base - sh);
+        const float veq = float(L::w[0]) * (s * base - sh * usq);
         float r = v[0] + om * (veq - v[0]);
         if constexpr (GUO) r += gc * float(L::w[0]) * (-3.f * uF);
         v[0] = r;
@@ -190,7 +196,8 @@ __device__ __forceinline__ void fast_relax(float *v, const FastState<L> &st, flo
         if constexpr (i < o) {
             constexpr float w = float(L::w[i]);
             const float eu = edot<L, i>(u3);
-            const float even = w * (rho * (base + 0.5f * eu * eu) - sh);
+            const float h = 0.5f * eu * eu;
+            const float even = w * (s * (base + h) + sh * (h - usq));
             const float odd = w * rho * eu;
             float ri = v[i] + om * ((even + odd) - v[i]);
             float ro = v[o] + om * ((even - odd) - v[o]);

@@ -108,17 +108,32 @@ __device__ __forceinline__ void node_update(const StepArgs<typename St::C> &a, i
     // ---- node class
     const uint16_t desc = m ? a.cls[m] : (uint16_t)0;
 
+    bool stored_done = false;
     if (desc & CLS_SOLID) {
         // bounceback, collision.cu:423-443
-        C t[Q];
-        sfor<0, Q>([&](auto ii) {
-            constexpr int i = decltype(ii)::value;
-            t[i] = f[L::opp[i]];
-        });
-        sfor<0, Q>([&](auto ii) {
-            constexpr int i = decltype(ii)::value;
-            f[i] = t[i];
-        });
+        if constexpr (St::shifted) {
+            // w_opp(i) = w_i: permute the stored values themselves (exact)
+            C t[Q];
+            sfor<0, Q>([&](auto ii) {
+                constexpr int i = decltype(ii)::value;
+                t[i] = v[L::opp[i]];
+            });
+            sfor<0, Q>([&](auto ii) {
+                constexpr int i = decltype(ii)::value;
+                v[i] = t[i];
+            });
+            stored_done = true;
+        } else {
+            C t[Q];
+            sfor<0, Q>([&](auto ii) {
+                constexpr int i = decltype(ii)::value;
+                t[i] = f[L::opp[i]];
+            });
+            sfor<0, Q>([&](auto ii) {
+                constexpr int i = decltype(ii)::value;
+                f[i] = t[i];
+            });
+        }
         rho = C(0);
         sfor<0, D>([&](auto dd) { u[decltype(dd)::value] = C(0); });
     } else {
@@ -233,11 +248,13 @@ __device__ __forceinline__ void node_update(const StepArgs<typename St::C> &a, i
         }
     }
 
-    sfor<0, Q>([&](auto ii) {
-        constexpr int i = decltype(ii)::value;
-        if constexpr (St::shifted) v[i] = f[i] - C(L::w[i]);
-        else v[i] = f[i];
-    });
+    if (!stored_done) {
+        sfor<0, Q>([&](auto ii) {
+            constexpr int i = decltype(ii)::value;
+            if constexpr (St::shifted) v[i] = f[i] - C(L::w[i]);
+            else v[i] = f[i];
+        });
+    }
     if (a.store) {
         sfor<0, D>([&](auto dd) {
             constexpr int d = decltype(dd)::value;

@@ -104,6 +104,7 @@ class ClosureOracle:
             xn = ((x.astype(np.float32) - self.mu) * self.isd).astype(np.float64)
             y, _ = pymlp.forward(self.p, xn, quant=self.quant)
             t = self.out_a * y + self.out_b
+            self.tau_net = t
             bad = ~((t >= self.lo) & (t <= self.hi))
             self.record = np.where(bad, np.nan, t)
         else:

@@ -113,9 +113,7 @@ def test_training_fits_and_is_deterministic():
     assert np.mean((y - t) ** 2) < 0.1 * np.var(t)
 
 
-def _closure_case(lattice, L, tau, c_smag, storage, force=None, solid=None, seed=20, steps=12, every=1):
-    """Random network whose tau' falls inside the band at most nodes and
-    outside it at some; returns (device macros, oracle macros, fallback share)."""
+def _initial_state(lattice, L, seed):
     dim = 2 if lattice == lbmc.D2Q9 else 3
     n = int(np.prod(L))
     rng = np.random.default_rng(seed)
@@ -125,13 +123,20 @@ def _closure_case(lattice, L, tau, c_smag, storage, force=None, solid=None, seed
     else:
         u = 0.03 * rng.standard_normal((2, n))
     rho = 1.0 + 0.002 * rng.standard_normal(n)
+    return rho, u
+
+
+# input normalisation from the initial non-equilibrium scale
+MU = np.array([0, 0, 0, 0, 0, 0, 1.0], dtype=np.float32)
+ISD = np.array([300.0] * 6 + [100.0], dtype=np.float32)
+
+
+def _pair(lattice, L, tau, c_smag, storage, band, out, force=None, solid=None, seed=20, every=1):
+    """The GPU lattice and the oracle, same initial state and network."""
+    dim = 2 if lattice == lbmc.D2Q9 else 3
+    n = int(np.prod(L))
+    rho, u = _initial_state(lattice, L, seed)
     p = random_params(seed)
-    # normalisation from the initial non-equilibrium scale
-    mu = np.zeros(7, dtype=np.float32)
-    mu[6] = 1.0
-    isd = np.full(7, 300.0, dtype=np.float32)
-    isd[6] = 100.0
-    out_a, out_b, lo, hi = 0.02, tau + 0.01, tau, tau + 0.05
     lat = lbmc.Lattice(lattice, L, lbmc.BGK, forcing=force is not None, tau=tau, c_smag=c_smag,
                        storage=storage)
     fields = {}
@@ -141,42 +146,79 @@ def _closure_case(lattice, L, tau, c_smag, storage, force=None, solid=None, seed
         fields["force"] = [np.full(n, force[d]) for d in range(dim)]
     lat.set_fields(**fields)
     lat.init_equilibrium(rho, u)
-    m = lbmc.Mlp(pymlp.flatten(p), mu=mu, isd=isd, out_a=out_a, out_b=out_b)
-    lat.set_closure(m, lo, hi)
+    m = lbmc.Mlp(pymlp.flatten(p), mu=MU, isd=ISD, out_a=out[0], out_b=out[1])
+    lat.set_closure(m, band[0], band[1])
     if every > 1:
         lat.set_closure_interval(every)
-    lat.run(0, steps, 1)
-    rd, ud = lat.get_macros()
-    co = pyclosure.ClosureOracle(lattice, L, tau, c_smag, p, mu, isd, out_a, out_b, lo, hi,
+    co = pyclosure.ClosureOracle(lattice, L, tau, c_smag, p, MU, ISD, out[0], out[1], band[0], band[1],
                                  force=force, solid=solid, every=every)
     co.init_equilibrium(rho, u)
-    ro, uo = co.run(steps)
+    return lat, m, co
+
+
+@pytest.mark.parametrize("lattice", [lbmc.D3Q15, lbmc.D2Q9])
+def test_closure_single_call_matches_oracle(lattice):
+    """Call 0 (collision in place): post-collision populations at every node
+    whose network tau is not within 1e-3 of a band edge (where a last-bit
+    difference may pick the other branch), both branches present."""
+    print(f"[closure] single call lattice={lattice}", flush=True)
+    if lattice == lbmc.D3Q15:
+        L, tau, cs, force, solid = (16, 16, 16), 0.51, 0.17, None, None
+    else:
+        L, tau, cs, force = (40, 24), 0.6, 0.1, [2e-5, 0.0]
+        g = np.zeros((24, 40))
+        g[0, :] = g[-1, :] = 1
+        g[10:14, 18:22] = 1
+        solid = g.reshape(-1)
+    band = (tau + 0.005, tau + 0.04)
+    lat, m, co = _pair(lattice, L, tau, cs, lbmc.FP32, band, (0.02, tau + 0.02), force=force, solid=solid)
+    lat.step(0, True)
+    fd = lat.get_f()
+    co.step()
+    fo = co.f
+    edge = np.minimum(np.abs(co.tau_net - band[0]), np.abs(co.tau_net - band[1])) < 1e-3
+    keep = ~edge
+    fluid = keep & ~co.solid
+    assert co.fallback[fluid].sum() > 10 and (~co.fallback[fluid]).sum() > 10
+    assert np.max(np.abs(fd[:, keep] - fo[:, keep])) < 5e-6
     lat.close()
     m.close()
-    return (rd, ud), (ro, uo), co.fallback.mean()
 
 
-@pytest.mark.parametrize("storage,tol", [(lbmc.FP32, 2e-6), (lbmc.FP16S, 2e-4), (lbmc.BF16S, 2e-3)])
-def test_closure_step_d3q15_matches_oracle(storage, tol):
-    print(f"[closure] step D3Q15 16^3 storage={storage}", flush=True)
-    (rd, ud), (ro, uo), fb = _closure_case(lbmc.D3Q15, (16, 16, 16), 0.51, 0.17, storage)
-    assert 0.02 < fb < 0.9, fb  # both branches taken
+def _run_pair(lattice, L, tau, c_smag, storage, steps, force=None, solid=None, every=1):
+    """Network output well inside the band everywhere (no branch near its
+    edges): the macros after `steps` calls."""
+    band = (tau + 0.001, tau + 0.06)
+    lat, m, co = _pair(lattice, L, tau, c_smag, storage, band, (0.004, tau + 0.03), force=force, solid=solid,
+                       every=every)
+    lat.run(0, steps, 1)
+    rd, ud = lat.get_macros()
+    ro, uo = co.run(steps)
+    edge = np.minimum(np.abs(co.tau_net - band[0]), np.abs(co.tau_net - band[1]))
+    lat.close()
+    m.close()
+    return (rd, ud), (ro, uo), co.fallback.mean(), edge.min()
+
+
+@pytest.mark.parametrize("storage,tol", [(lbmc.FP32, 1e-5), (lbmc.FP16S, 2e-4), (lbmc.BF16S, 2e-3)])
+def test_closure_steps_d3q15_match_oracle(storage, tol):
+    print(f"[closure] 12 calls D3Q15 16^3 storage={storage}", flush=True)
+    (rd, ud), (ro, uo), fb, edge = _run_pair(lbmc.D3Q15, (16, 16, 16), 0.51, 0.17, storage, 12)
+    assert fb == 0.0 and edge > 1e-3
     assert np.max(np.abs(ud - uo)) < tol
     assert np.max(np.abs(rd - ro)) < tol
 
 
-def test_closure_step_d2q9_guo_solids_matches_oracle():
-    print("[closure] step D2Q9 Guo + solids", flush=True)
-    Lx, Ly = 40, 24
-    solid = np.zeros((Ly, Lx))
-    solid[0, :] = 1
-    solid[-1, :] = 1
-    solid[10:14, 18:22] = 1
-    (rd, ud), (ro, uo), fb = _closure_case(lbmc.D2Q9, (Lx, Ly), 0.6, 0.1, lbmc.FP32,
-                                           force=[2e-5, 0.0], solid=solid.reshape(-1), steps=15)
-    assert 0.02 < fb < 0.98, fb
-    assert np.max(np.abs(ud - uo)) < 2e-6
-    assert np.max(np.abs(rd - ro)) < 2e-6
+def test_closure_steps_d2q9_guo_solids_match_oracle():
+    print("[closure] 15 calls D2Q9 Guo + solids", flush=True)
+    g = np.zeros((24, 40))
+    g[0, :] = g[-1, :] = 1
+    g[10:14, 18:22] = 1
+    (rd, ud), (ro, uo), fb, edge = _run_pair(lbmc.D2Q9, (40, 24), 0.6, 0.1, lbmc.FP32, 15, force=[2e-5, 0.0],
+                                             solid=g.reshape(-1))
+    assert edge > 1e-3
+    assert np.max(np.abs(ud - uo)) < 1e-5
+    assert np.max(np.abs(rd - ro)) < 1e-5
 
 
 @pytest.mark.parametrize("every", [2, 3])
@@ -184,10 +226,10 @@ def test_closure_interval_matches_oracle(every):
     # 16^3: after the first (evaluating) call, the remaining 99 calls cover
     # the captured period lcm(16, every) at least twice, so graph replays run
     print(f"[closure] interval {every}", flush=True)
-    (rd, ud), (ro, uo), fb = _closure_case(lbmc.D3Q15, (16, 16, 16), 0.51, 0.17, lbmc.FP32, steps=100,
-                                           every=every)
-    assert np.max(np.abs(ud - uo)) < 1e-5
-    assert np.max(np.abs(rd - ro)) < 1e-5
+    (rd, ud), (ro, uo), fb, edge = _run_pair(lbmc.D3Q15, (16, 16, 16), 0.51, 0.17, lbmc.FP32, 100, every=every)
+    assert edge > 1e-3
+    assert np.max(np.abs(ud - uo)) < 4e-5
+    assert np.max(np.abs(rd - ro)) < 4e-5
 
 
 def test_closure_off_restores_smagorinsky_bitwise():
