@@ -11,8 +11,9 @@ coarse run from another initial field.
      library's kernels;
   3. coarse runs (grid / r, tau_c = 1/2 + (tau_f - 1/2) / r, r fine steps per
      coarse step) from the box-filtered test field, with Smagorinsky
-     (C_s = 0.17) and with the learned closure (Smagorinsky outside
-     [0.5005, 2.0]), and without any model;
+     (C_s = 0.17), with the learned closure (Smagorinsky outside
+     [0.5005, 2.0]) evaluated every call and every --every calls, and
+     without any model;
   4. shell-averaged energy spectra of each coarse run against the
      box-filtered fine test run at the same physical times; error =
      mean over k in [1, k_max] of |log10(E_coarse / E_ref)|.
@@ -82,6 +83,7 @@ def main():
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--c-smag", type=float, default=0.17)
+    ap.add_argument("--every", type=int, default=4, help="closure evaluation interval of the second learned run")
     ap.add_argument("--out", type=str, default="closure_experiment.json")
     args = ap.parse_args()
 
@@ -152,13 +154,16 @@ def main():
 
     # ---- 4. coarse runs
     runs = {}
-    for name in ("smagorinsky", "learned", "no_model"):
+    ap_every = args.every
+    for name in ("smagorinsky", "learned", f"learned_every_{ap_every}", "no_model"):
         c = lbmc.Lattice(lbmc.D3Q15, (Lc, Lc, Lc), lbmc.BGK, tau=tau_c,
                          c_smag=0.0 if name == "no_model" else args.c_smag, storage=lbmc.FP32)
         c.set_fields()
         c.init_equilibrium(np.ones(Lc ** 3), tuple(v.reshape(-1) for v in ub0))
-        if name == "learned":
+        if name.startswith("learned"):
             c.set_closure(model, 0.5005, 2.0)
+            if name != "learned":
+                c.set_closure_interval(ap_every)
         errs, tc = {}, 0
         for ts in evals:
             n = ts // r
