@@ -158,9 +158,11 @@ def _pair(lattice, L, tau, c_smag, storage, band, out, force=None, solid=None, s
 
 @pytest.mark.parametrize("lattice", [lbmc.D3Q15, lbmc.D2Q9])
 def test_closure_single_call_matches_oracle(lattice):
-    """Call 0 (collision in place): post-collision populations at every node
-    whose network tau is not within 1e-3 of a band edge (where a last-bit
-    difference may pick the other branch), both branches present."""
+    """Call 0 (collision in place) from equilibrium plus a random second-order
+    non-equilibrium part f_i^neq = 4.5 w_i (e_i e_i - I/3) : A (zero mass and
+    momentum, Pi = A): post-collision populations at every node whose network
+    tau is not within 1e-3 of a band edge (where a last-bit difference may
+    pick the other branch), with both branches present."""
     print(f"[closure] single call lattice={lattice}", flush=True)
     if lattice == lbmc.D3Q15:
         L, tau, cs, force, solid = (16, 16, 16), 0.51, 0.17, None, None
@@ -172,6 +174,16 @@ def test_closure_single_call_matches_oracle(lattice):
         solid = g.reshape(-1)
     band = (tau + 0.005, tau + 0.04)
     lat, m, co = _pair(lattice, L, tau, cs, lbmc.FP32, band, (0.02, tau + 0.02), force=force, solid=solid)
+    dim = co.dim
+    n = int(np.prod(L))
+    rng = np.random.default_rng(77)
+    A = 2e-3 * rng.standard_normal((dim, dim, n))
+    A = 0.5 * (A + A.transpose(1, 0, 2))
+    Q = np.einsum("ai,bi->abi", co.e, co.e) - np.eye(dim)[:, :, None] / 3.0       # [a, b, i]
+    fneq = 4.5 * co.w[:, None] * np.einsum("abi,abn->in", Q, A)
+    f = co.f + fneq
+    lat.set_f(f)
+    co.f = f.copy()
     lat.step(0, True)
     fd = lat.get_f()
     co.step()
@@ -179,7 +191,7 @@ def test_closure_single_call_matches_oracle(lattice):
     edge = np.minimum(np.abs(co.tau_net - band[0]), np.abs(co.tau_net - band[1])) < 1e-3
     keep = ~edge
     fluid = keep & ~co.solid
-    assert co.fallback[fluid].sum() > 10 and (~co.fallback[fluid]).sum() > 10
+    assert co.fallback[fluid].sum() > 10 and (~co.fallback[fluid]).sum() > 10, co.fallback[fluid].mean()
     assert np.max(np.abs(fd[:, keep] - fo[:, keep])) < 5e-6
     lat.close()
     m.close()
