@@ -217,6 +217,14 @@ static double u_square(const orc_lattice *lt, const onode *nd)
     return v * 1.5;
 }
 
+/* ORC_MUTANT (default 0) plants one deliberate mistake per value, for
+ * tests/test_oracle_pins_bite.py, which builds each mutant into a scratch
+ * directory and checks that the pin written for that function fails on it.
+ * The library built by oracle/Makefile has none. */
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+
 /* e_mul_u, collision.cu:36-47 */
 static double e_mul_u(const orc_lattice *lt, const onode *nd, int i)
 {
@@ -231,7 +239,11 @@ static void equilibrium(const orc_lattice *lt, const onode *nd, double *feq)
     double usq = u_square(lt, nd);
     for (int i = 0; i < lt->q; i++) {
         double eu = e_mul_u(lt, nd, i);
+#if ORC_MUTANT == 1
+        feq[i] = nd->rho * lt->w[i] * (1.0 + eu - usq); /* second-order term dropped */
+#else
         feq[i] = nd->rho * lt->w[i] * (1.0 + eu + (0.5 * eu * eu) - usq);
+#endif
     }
 }
 
@@ -254,7 +266,11 @@ static void turbulent_viscosity(const orc_lattice *lt, const onode *nd, const do
     double s3 = sqrt((double)3);
     double den = nd->rho * 1 / s3 * 1 / s3 * 1 / s3 * 1 / s3;
     double c = nd->c_smag;
+#if ORC_MUTANT == 3
+    *tau = *tau + 0.5 * (sqrt(((*tau) * (*tau)) + (sqrt((double)2) * (c * c) * (1 / den) * q_hat)) - *tau);
+#else
     *tau = *tau + 0.5 * (sqrt(((*tau) * (*tau)) + (2 * sqrt((double)2) * (c * c) * (1 / den) * q_hat)) - *tau);
+#endif
 }
 
 /* Guo source term, collision.cu:82-99 (uses the node's shifted u and the
@@ -269,8 +285,13 @@ static void guo_source(const orc_lattice *lt, const onode *nd, double tau, doubl
                 eu = (lt->e[0][i] * nd->u[0]) + (lt->e[1][i] * nd->u[1]) + (lt->e[2][i] * nd->u[2]);
             else
                 eu = (lt->e[0][i] * nd->u[0]) + (lt->e[1][i] * nd->u[1]);
+#if ORC_MUTANT == 2
+            Fc[d] = lt->w[i] * (1 - (1 / (2 * tau))) *
+                    (((lt->e[d][i] - nd->u[d]) * 3) + (lt->e[d][i] * 3 * eu)); /* 9 -> 3 */
+#else
             Fc[d] = lt->w[i] * (1 - (1 / (2 * tau))) *
                     (((lt->e[d][i] - nd->u[d]) * 3) + (lt->e[d][i] * 9 * eu));
+#endif
         }
         ft[i] = 0;
         for (int d = 0; d < lt->dim; d++) ft[i] += Fc[d] * nd->F[d];
@@ -280,7 +301,11 @@ static void guo_source(const orc_lattice *lt, const onode *nd, double tau, doubl
 /* u <- u + F/(2 rho), collision.cu:74-78 */
 static void guo_shift(const orc_lattice *lt, onode *nd)
 {
+#if ORC_MUTANT == 6
+    for (int d = 0; d < lt->dim; d++) nd->u[d] = nd->u[d] + (1 / nd->rho) * nd->F[d]; /* F/rho */
+#else
     for (int d = 0; d < lt->dim; d++) nd->u[d] = nd->u[d] + (1 / (2 * nd->rho)) * nd->F[d];
+#endif
 }
 
 /* meq_d2q9 / meq_d3q15, collision.cu:465-503 */
@@ -291,7 +316,11 @@ static void moment_equilibrium(const orc_lattice *lt, unsigned quirks, const ono
         double jx = nd->rho * nd->u[0];
         double jy = nd->rho * nd->u[1];
         meq[0] = nd->rho;
+#if ORC_MUTANT == 7
+        meq[1] = (2 * nd->rho) + 3 * (jx * jx + jy * jy); /* sign of rho */
+#else
         meq[1] = (-2 * nd->rho) + 3 * (jx * jx + jy * jy);
+#endif
         meq[2] = nd->rho - 3 * (jx * jx + jy * jy);
         meq[3] = jx;
         meq[4] = -jx;
@@ -418,7 +447,11 @@ static void collide(const orc_model *m, int code, onode *nd, double *tau)
         break;
     default: /* bounceback, collision.cu:423-443 */
         for (int i = 0; i < Q; i++) tmp[i] = nd->f[i];
+#if ORC_MUTANT == 5
+        for (int i = 0; i < Q; i++) nd->f[i] = tmp[i]; /* no reflection */
+#else
         for (int i = 0; i < Q; i++) nd->f[i] = tmp[lt->opp[i]];
+#endif
         nd->u[0] = 0;
         nd->u[1] = 0;
         if (lt->dim > 2) nd->u[2] = 0;
@@ -474,7 +507,11 @@ static void zh_d2q9(int east, onode *nd)
         nd->u[0] = 1.0 - ((1.0 / r) * (f[0] + f[2] + f[4] + 2.0 * (f[3] + f[6] + f[7])));
         nd->u[1] = 0.0;
         f[1] = f[3] + ((2.0 / 3.0) * r * nd->u[0]);
+#if ORC_MUTANT == 4
+        f[5] = f[7] + ((1.0 / 2.0) * (f[2] - f[4])) + ((1.0 / 6.0) * r * nd->u[0]); /* sign */
+#else
         f[5] = f[7] - ((1.0 / 2.0) * (f[2] - f[4])) + ((1.0 / 6.0) * r * nd->u[0]);
+#endif
         f[8] = f[6] + ((1.0 / 2.0) * (f[2] - f[4])) + ((1.0 / 6.0) * r * nd->u[0]);
     } else {
         nd->u[0] = -1.0 + ((1.0 / r) * (f[0] + f[2] + f[4] + 2.0 * (f[1] + f[5] + f[8])));
@@ -754,7 +791,11 @@ static long slot_of(const orc_model *m, const int *coord, long long t, int i)
 {
     long ix[3] = {0, 0, 0};
     for (int d = 0; d < m->lat.dim; d++) {
+#if ORC_MUTANT == 8
+        long long v = ((long long)coord[d] + (t * m->lat.e[d][i])) % m->L[d]; /* shift sign */
+#else
         long long v = ((long long)coord[d] - (t * m->lat.e[d][i])) % m->L[d];
+#endif
         if (v < 0) v = m->L[d] - (v * -1);
         ix[d] = (long)v;
     }

@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liblbmc_oracle.so")
+# LBMC_ORACLE_LIB selects another build (the planted-mistake builds of
+# tests/test_oracle_pins_bite.py)
+LIB_PATH = os.environ.get("LBMC_ORACLE_LIB") or os.path.join(HERE, "liblbmc_oracle.so")
 
 D2Q9, D3Q15 = 0, 1
 BGK, NTPOR, MRT, MRTPOR = 0, 1, 2, 3
