@@ -40,9 +40,9 @@ def log(msg):
     print(f"[closure-exp {time.strftime('%H:%M:%S')}] {msg}", flush=True)
 
 
-def fine_lattice(L, tau, seed, k0, u_rms, storage):
+def fine_lattice(L, tau, seed, k0, u_rms, storage, c_smag=0.0):
     ux, uy, uz = workloads.decaying_turbulence(L, k0=k0, u_rms=u_rms, seed=seed)
-    lat = lbmc.Lattice(lbmc.D3Q15, (L, L, L), lbmc.BGK, tau=tau, storage=storage)
+    lat = lbmc.Lattice(lbmc.D3Q15, (L, L, L), lbmc.BGK, tau=tau, c_smag=c_smag, storage=storage)
     lat.set_fields()
     lat.init_equilibrium(np.ones(L ** 3), (ux.reshape(-1), uy.reshape(-1), uz.reshape(-1)))
     return lat, (ux, uy, uz)
@@ -84,6 +84,8 @@ def main():
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--c-smag", type=float, default=0.17)
     ap.add_argument("--every", type=int, default=4, help="closure evaluation interval of the second learned run")
+    ap.add_argument("--fine-c-smag", type=float, default=0.0,
+                    help="Smagorinsky constant of the fine runs (0: plain BGK)")
     ap.add_argument("--out", type=str, default="closure_experiment.json")
     args = ap.parse_args()
 
@@ -94,13 +96,14 @@ def main():
     evals = [int(s) for s in args.eval.split(",")]
     kmax = Lc // 4
     res = {"fine": Lf, "coarse": Lc, "ratio": r, "tau_fine": args.tau_fine, "tau_coarse": tau_c,
+           "fine_c_smag": args.fine_c_smag,
            "k0": args.k0, "u_rms": args.u_rms, "c_smag": args.c_smag, "kmax": kmax}
 
     # ---- 1-2. training data from the resolved training run
     rng = np.random.default_rng(args.train_seed)
     per_snap = max(1, args.samples // len(snaps))
     feats, targs = [], []
-    lat, _ = fine_lattice(Lf, args.tau_fine, args.train_seed, args.k0, args.u_rms, lbmc.FP32)
+    lat, _ = fine_lattice(Lf, args.tau_fine, args.train_seed, args.k0, args.u_rms, lbmc.FP32, args.fine_c_smag)
     t = 0
     t0 = time.perf_counter()
     for ts in snaps:
@@ -139,7 +142,8 @@ def main():
     log(f"trained on {X.shape[0]} samples: loss {hist[0]:.4f} -> {hist[-1]:.4f} (normalised target variance 1)")
 
     # ---- 3. resolved test run: filtered reference spectra and the coarse initial field
-    lat, (ux, uy, uz) = fine_lattice(Lf, args.tau_fine, args.test_seed, args.k0, args.u_rms, lbmc.FP32)
+    lat, (ux, uy, uz) = fine_lattice(Lf, args.tau_fine, args.test_seed, args.k0, args.u_rms, lbmc.FP32,
+                                     args.fine_c_smag)
     ub0 = np.stack([closure.box_filter(v, r) for v in (ux, uy, uz)])
     ref = {}
     t = 0
