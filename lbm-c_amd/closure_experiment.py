@@ -72,12 +72,14 @@ def main():
     ap.add_argument("--ratio", type=int, default=4)
     ap.add_argument("--tau-fine", type=float, default=0.505)
     ap.add_argument("--k0", type=float, default=4.0)
-    ap.add_argument("--u-rms", type=float, default=0.05)
+    ap.add_argument("--u-rms", type=float, default=0.01,
+                    help="per-component rms velocity of the initial field; 0.01 keeps the fine run "
+                         "resolved (Kolmogorov scale ~0.5 fine nodes at 512^3, k0 = 4)")
     ap.add_argument("--train-seed", type=int, default=11)
     ap.add_argument("--test-seed", type=int, default=2024)
-    ap.add_argument("--snapshots", type=str, default="1000,2000,3000,4000,6000,8000",
+    ap.add_argument("--snapshots", type=str, default="2000,4000,6000,8000,10000,12000",
                     help="fine steps of the training snapshots")
-    ap.add_argument("--eval", type=str, default="2000,4000,8000", help="fine steps of the comparisons")
+    ap.add_argument("--eval", type=str, default="4000,8000,12000", help="fine steps of the comparisons")
     ap.add_argument("--samples", type=int, default=4_000_000)
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--batch", type=int, default=4096)
