@@ -86,6 +86,8 @@ def main():
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--c-smag", type=float, default=0.17)
     ap.add_argument("--every", type=int, default=4, help="closure evaluation interval of the second learned run")
+    ap.add_argument("--smag-sweep", type=str, default="0.06,0.1",
+                    help="further Smagorinsky constants run as baselines")
     ap.add_argument("--fine-c-smag", type=float, default=0.0,
                     help="Smagorinsky constant of the fine runs (0: plain BGK)")
     ap.add_argument("--out", type=str, default="closure_experiment.json")
@@ -161,9 +163,15 @@ def main():
     # ---- 4. coarse runs
     runs = {}
     ap_every = args.every
-    for name in ("smagorinsky", "learned", f"learned_every_{ap_every}", "no_model"):
-        c = lbmc.Lattice(lbmc.D3Q15, (Lc, Lc, Lc), lbmc.BGK, tau=tau_c,
-                         c_smag=0.0 if name == "no_model" else args.c_smag, storage=lbmc.FP32)
+    sweep = [float(v) for v in args.smag_sweep.split(",") if v]
+    names = ["smagorinsky", "learned", f"learned_every_{ap_every}", "no_model"] + [f"smagorinsky_cs{v:g}" for v in sweep]
+    for name in names:
+        cs = args.c_smag
+        if name == "no_model":
+            cs = 0.0
+        elif name.startswith("smagorinsky_cs"):
+            cs = float(name[len("smagorinsky_cs"):])
+        c = lbmc.Lattice(lbmc.D3Q15, (Lc, Lc, Lc), lbmc.BGK, tau=tau_c, c_smag=cs, storage=lbmc.FP32)
         c.set_fields()
         c.init_equilibrium(np.ones(Lc ** 3), tuple(v.reshape(-1) for v in ub0))
         if name.startswith("learned"):
