@@ -258,6 +258,50 @@ __device__ __forceinline__ void moment_equilibrium(C rho, const C *u, C qc, C *m
     }
 }
 
+// Shifted storage (g = f - w): M g - (meq - M w) equals M f - meq, and
+// M w is the equilibrium's moment vector at rest with rho = 1, so meq - M w
+// is meq with rho replaced by s = rho - 1 in its rho-linear terms.  This
+// keeps the MRT update on g free of the rounding of rho and of f = g + w.
+template <class L>
+constexpr bool mw_is_rest_equilibrium()
+{
+    // M w must be (1, -2, 1, 0, ...) for D2Q9 and (1, -1, 1, 0, ...) for D3Q15
+    for (int k = 0; k < L::Q; k++) {
+        double acc = 0;
+        for (int j = 0; j < L::Q; j++) acc += L::M[k][j] * L::w[j];
+        double want = 0;
+        if (k == 0 || k == 2) want = 1;
+        if (k == 1) want = L::D == 2 ? -2 : -1;
+        const double d = acc - want;
+        if (d > 1e-12 || d < -1e-12) return false;
+    }
+    return true;
+}
+static_assert(mw_is_rest_equilibrium<D2Q9>(), "D2Q9 M w");
+static_assert(mw_is_rest_equilibrium<D3Q15>(), "D3Q15 M w");
+
+template <class L, class C>
+__device__ __forceinline__ void moment_equilibrium_shifted(C s, C rho, const C *u, C qc, C *meq)
+{
+    moment_equilibrium<L>(rho, u, qc, meq);
+    if constexpr (L::D == 2) {
+        const C jx = rho * u[0];
+        const C jy = rho * u[1];
+        const C j2 = jx * jx + jy * jy;
+        meq[0] = s;
+        meq[1] = (C(-2) * s) + C(3) * j2;
+        meq[2] = s - C(3) * j2;
+    } else {
+        const C jx = rho * u[0];
+        const C jy = rho * u[1];
+        const C jz = rho * u[2];
+        const C j2 = jx * jx + jy * jy + jz * jz;
+        meq[0] = s;
+        meq[1] = (-s) + j2;
+        meq[2] = s - C(5) * j2;
+    }
+}
+
 // m_k = s_k (sum_j M_kj f_j - meq_k), collision.cu:204-216
 template <class L, class C>
 __device__ __forceinline__ void mrt_moments(const C *f, const C *meq, const C *s, C *m)

@@ -90,6 +90,7 @@ __device__ __forceinline__ void node_update(const StepArgs<typename St::C> &a, i
         if constexpr (St::shifted) f[i] = v[i] + C(L::w[i]);
         else f[i] = v[i];
     });
+    C srho = rho;  // rho - 1 for shifted storage, exactly as summed
     if constexpr (St::shifted) rho = C(1) + rho;
     // FP32 compute regroups the arithmetic (one reciprocal of rho, paired
     // equilibria, the closed-form Smagorinsky norm); FP64 keeps the
@@ -144,7 +145,10 @@ __device__ __forceinline__ void node_update(const StepArgs<typename St::C> &a, i
             if (mc) {
                 // presc_*, macro_bc.cu:19-125
                 const unsigned ps = macro_set(mc);
-                if (ps & 1u) rho = a.rho[n];
+                if (ps & 1u) {
+                    rho = a.rho[n];
+                    srho = St::shifted ? rho - C(1) : rho;
+                }
                 sfor<0, D>([&](auto dd) {
                     constexpr int d = decltype(dd)::value;
                     if (ps & (2u << d)) u[d] = a.u[d][n];
@@ -223,6 +227,34 @@ __device__ __forceinline__ void node_update(const StepArgs<typename St::C> &a, i
                 guo_source<L>(a.tau, u, F, ft);
             }
             C meq[Q], mm[Q];
+            bool gspace = false;
+            if constexpr (St::shifted) gspace = cls_micro(desc) == 0;
+            if (gspace) {
+                // on the stored g = f - w (no micro BC changed f): the same
+                // update, M g - (meq - M w) = M f - meq, and f[opp] - f =
+                // g[opp] - g because w_opp(i) = w_i
+                moment_equilibrium_shifted<L>(srho, rho, u, qc, meq);
+                mrt_moments<L>(v, meq, a.tau_mrt, mm);
+                if (a.porous) {
+                    const C ob = C(1) - B;
+                    sfor<0, Q>([&](auto ii) {
+                        constexpr int i = decltype(ii)::value;
+                        const C acc = minv_row<L, MFIX, i>(mm);
+                        const C cs = v[L::opp[i]] - v[i];
+                        C r = v[i] - ob * acc;
+                        if constexpr (GUO) r = r + ob * ft[i];
+                        v[i] = r + B * cs;
+                    });
+                } else {
+                    sfor<0, Q>([&](auto ii) {
+                        constexpr int i = decltype(ii)::value;
+                        C r = v[i] - minv_row<L, MFIX, i>(mm);
+                        if constexpr (GUO) r = r + ft[i];
+                        v[i] = r;
+                    });
+                }
+                stored_done = true;
+            } else {
             moment_equilibrium<L>(rho, u, qc, meq);
             mrt_moments<L>(f, meq, a.tau_mrt, mm);
             if (a.porous) {
@@ -244,6 +276,7 @@ __device__ __forceinline__ void node_update(const StepArgs<typename St::C> &a, i
                     if constexpr (GUO) r = r + ft[i];
                     f[i] = r;
                 });
+            }
             }
         }
     }
