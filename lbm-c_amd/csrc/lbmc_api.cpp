@@ -120,7 +120,7 @@ lbmc_status launch_step(lbmc_ctx *c, long long t, int store)
                 a.tmode = (!c->tcache_valid || t % c->closure_every == 0) ? 1 : 2;
                 if (a.tmode == 1) c->tcache_valid = true;
             }
-            if (a.tmode != 2) grid = c->closure_grid;
+            grid = a.tmode != 2 ? c->closure_grid : c->reuse_grid;
         }
         switch (c->storage) {
         case LBMC_FP32: e = launch_step_fp32(c->key, a, grid, c->stream); break;
@@ -137,7 +137,12 @@ lbmc_status launch_step(lbmc_ctx *c, long long t, int store)
 // contiguous range of nodes (LBMC_XCD_REMAP=1 turns it on).
 void choose_shape(lbmc_ctx *c)
 {
-    c->grid = (int)((c->N + kBlock - 1) / kBlock);
+    // nodes per thread of the plain step kernel (LBMC_NPT=2: two nodes per
+    // thread; not for the 64-bit-offset variant)
+    c->npt = 1;
+    if (const char *env = getenv("LBMC_NPT"))
+        if (atoi(env) == 2 && !c->key.wide) c->npt = 2;
+    c->grid = (int)((c->N + (long long)kBlock * c->npt - 1) / ((long long)kBlock * c->npt));
     c->dx = make_fastdiv(c->L[0]);
     c->dy = make_fastdiv(c->L[1]);
     c->remap = 0;
@@ -148,7 +153,9 @@ void choose_shape(lbmc_ctx *c)
     int cap = 2048;
     if (const char *env = getenv("LBMC_CLOSURE_GRID"))
         if (atoi(env) > 0) cap = atoi(env);
-    c->closure_grid = std::min(c->grid, cap);
+    const int chunks = (int)((c->N + kBlock - 1) / kBlock);
+    c->closure_grid = std::min(chunks, cap);
+    c->reuse_grid = chunks;
 }
 
 void drop_graph(lbmc_ctx *c)
