@@ -56,7 +56,6 @@ struct lbmc_ctx {
 
     // launch shape
     int grid = 1, remap = 0;
-    int npt = 1;  // nodes per thread of the plain step kernel
     int closure_grid = 1;  // blocks of the grid-stride closure kernel
     int reuse_grid = 1;    // blocks of the closure-reuse kernel (one per 256 nodes)
     lbmc::FastDiv dx{}, dy{};
@@ -121,7 +120,6 @@ inline void fill_args(const lbmc_ctx *c, StepArgs<C> &a, long long t, int store)
     a.mlp = c->mlp_img;
     a.tcache = c->tcache;
     a.tmode = 0;
-    a.npt = c->npt;
 }
 
 }  // namespace lbmc

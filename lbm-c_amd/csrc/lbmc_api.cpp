@@ -137,12 +137,7 @@ lbmc_status launch_step(lbmc_ctx *c, long long t, int store)
 // contiguous range of nodes (LBMC_XCD_REMAP=1 turns it on).
 void choose_shape(lbmc_ctx *c)
 {
-    // nodes per thread of the plain step kernel (LBMC_NPT=2: two nodes per
-    // thread; not for the 64-bit-offset variant)
-    c->npt = 1;
-    if (const char *env = getenv("LBMC_NPT"))
-        if (atoi(env) == 2 && !c->key.wide) c->npt = 2;
-    c->grid = (int)((c->N + (long long)kBlock * c->npt - 1) / ((long long)kBlock * c->npt));
+    c->grid = (int)((c->N + kBlock - 1) / kBlock);
     c->dx = make_fastdiv(c->L[0]);
     c->dy = make_fastdiv(c->L[1]);
     c->remap = 0;

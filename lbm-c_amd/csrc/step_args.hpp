@@ -72,7 +72,6 @@ struct StepArgs {
     const void *mlp;         // closure weights (MlpImage) when the closure is on
     float *tcache;           // closure: per-node tau of the last evaluation (NaN: fall back)
     int tmode;               // closure: 0 evaluate, 1 evaluate and record, 2 use the record
-    int npt;                 // nodes per thread of the step kernel (1 or 2; 1 for wide arrays)
 };
 
 }  // namespace lbmc

@@ -379,10 +379,8 @@ __device__ __forceinline__ void slot_offsets(const StepArgs<C> &a, int n, int (&
 // One thread per node, nodes in slot order (x fastest).  For a fixed
 // direction, consecutive nodes read consecutive slots of the source row
 // (rotated by t*e_x), so every wave's accesses are contiguous apart from
-// at most one wrap point.  NPT = 2: each thread takes two nodes kBlock apart
-// (both waves' accesses stay contiguous) and issues both gathers before
-// either update, doubling the loads in flight per wave.
-template <class L, class St, int FAM, bool GUO, bool SMAG, bool MFIX, bool WIDE, int NPT>
+// at most one wrap point.
+template <class L, class St, int FAM, bool GUO, bool SMAG, bool MFIX, bool WIDE>
 __global__ __launch_bounds__(kBlock) void step_kernel(const StepArgs<typename St::C> a)
 {
     int b = (int)blockIdx.x;
@@ -393,37 +391,14 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const StepArgs<typename St
         const int g8 = (int)gridDim.x >> 3;
         b = (b & 7) * g8 + (b >> 3);
     }
-    if constexpr (NPT == 1) {
-        const int n = b * kBlock + (int)threadIdx.x;
-        if (n >= a.nnodes) return;
-        int dd[3][3];
-        slot_offsets<L>(a, n, dd);
-        // WIDE = false (direction arrays below 4 GiB): a uniform base per
-        // direction plus a 32-bit byte offset per lane, no 64-bit address
-        // arithmetic per access
-        step_node<L, St, FAM, GUO, SMAG, MFIX, WIDE>(a, n, dd);
-    } else {
-        using C = typename St::C;
-        const int first = b * (kBlock * NPT) + (int)threadIdx.x;
-        if (first >= a.nnodes) return;
-        C v[NPT][L::Q];
-        int dd[NPT][3][3];
-        int nn[NPT];
-        sfor<0, NPT>([&](auto kk) {
-            constexpr int k = decltype(kk)::value;
-            // past the last node: read the last node's slots, write nothing
-            nn[k] = min(first + k * kBlock, a.nnodes - 1);
-            slot_offsets<L>(a, nn[k], dd[k]);
-            gather_node<L, St, WIDE>(a, nn[k], dd[k], v[k]);
-        });
-        sfor<0, NPT>([&](auto kk) {
-            constexpr int k = decltype(kk)::value;
-            if (first + k * kBlock < a.nnodes) {
-                node_update<L, St, FAM, GUO, SMAG, MFIX>(a, nn[k], v[k]);
-                scatter_node<L, St, WIDE>(a, nn[k], dd[k], v[k]);
-            }
-        });
-    }
+    const int n = b * kBlock + (int)threadIdx.x;
+    if (n >= a.nnodes) return;
+    int dd[3][3];
+    slot_offsets<L>(a, n, dd);
+    // WIDE = false (direction arrays below 4 GiB): a uniform base per
+    // direction plus a 32-bit byte offset per lane, no 64-bit address
+    // arithmetic per access
+    step_node<L, St, FAM, GUO, SMAG, MFIX, WIDE>(a, n, dd);
 }
 
 // Gathers the post-stream populations of the neighbours of the sf nodes

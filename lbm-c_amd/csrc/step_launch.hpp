@@ -14,13 +14,10 @@ namespace lbmc {
 template <class L, class St, int FAM, bool GUO, bool SMAG, bool MFIX>
 static hipError_t launch_one(const StepArgs<typename St::C> &a, int grid, hipStream_t s, bool wide)
 {
-    // grid covers kBlock * a.npt nodes per block
     if (wide)
-        hipLaunchKernelGGL((step_kernel<L, St, FAM, GUO, SMAG, MFIX, true, 1>), dim3(grid), dim3(kBlock), 0, s, a);
-    else if (a.npt == 2)
-        hipLaunchKernelGGL((step_kernel<L, St, FAM, GUO, SMAG, MFIX, false, 2>), dim3(grid), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL((step_kernel<L, St, FAM, GUO, SMAG, MFIX, true>), dim3(grid), dim3(kBlock), 0, s, a);
     else
-        hipLaunchKernelGGL((step_kernel<L, St, FAM, GUO, SMAG, MFIX, false, 1>), dim3(grid), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL((step_kernel<L, St, FAM, GUO, SMAG, MFIX, false>), dim3(grid), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }
 
