@@ -161,7 +161,7 @@ def main():
         isd[6] = 1e2
         model = lbmc.Mlp(closure.init_params(7), mu=mu, isd=isd, out_a=0.005, out_b=0.515)
         closure_line = {"mfma_flop_per_node": 2 * (16 * 64 + 64 * 64 + 64 * 16)}
-        for every in (1, 2, 4):
+        for every in (1, 2, 4, 8):
             c = make_lattice(lbmc, L, lbmc.FP32, fields)
             c.set_closure(model)
             c.set_closure_interval(every)

@@ -85,7 +85,7 @@ def main():
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--c-smag", type=float, default=0.17)
-    ap.add_argument("--every", type=int, default=4, help="closure evaluation interval of the second learned run")
+    ap.add_argument("--every", type=str, default="4,8", help="closure evaluation intervals of the further learned runs")
     ap.add_argument("--smag-sweep", type=str, default="0.06,0.1",
                     help="further Smagorinsky constants run as baselines")
     ap.add_argument("--fine-c-smag", type=float, default=0.0,
@@ -162,9 +162,10 @@ def main():
 
     # ---- 4. coarse runs
     runs = {}
-    ap_every = args.every
+    everys = [int(v) for v in args.every.split(",") if v]
     sweep = [float(v) for v in args.smag_sweep.split(",") if v]
-    names = ["smagorinsky", "learned", f"learned_every_{ap_every}", "no_model"] + [f"smagorinsky_cs{v:g}" for v in sweep]
+    names = (["smagorinsky", "learned"] + [f"learned_every_{k}" for k in everys] + ["no_model"]
+             + [f"smagorinsky_cs{v:g}" for v in sweep])
     for name in names:
         cs = args.c_smag
         if name == "no_model":
@@ -177,7 +178,7 @@ def main():
         if name.startswith("learned"):
             c.set_closure(model, 0.5005, 2.0)
             if name != "learned":
-                c.set_closure_interval(ap_every)
+                c.set_closure_interval(int(name[len("learned_every_"):]))
         errs, tc = {}, 0
         for ts in evals:
             n = ts // r
