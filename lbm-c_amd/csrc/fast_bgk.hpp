@@ -120,7 +120,6 @@ __device__ __forceinline__ void fast_prepare(const float *v, const float *F, Fas
 template <class L, bool SH>
 __device__ __forceinline__ float fast_smag_tau(const FastState<L> &st, float tau, float c_smag)
 {
-    constexpr float sh = SH ? 1.f : 0.f;
     const float rho = st.rho;
     const float *u = st.u;
     const float *P = st.P;
@@ -141,7 +140,6 @@ __device__ __forceinline__ float fast_smag_tau(const FastState<L> &st, float tau
 template <class L, bool SH>
 __device__ __forceinline__ void fast_closure_features(const FastState<L> &st, float (&x)[7])
 {
-    constexpr float sh = SH ? 1.f : 0.f;
     const float rho = st.rho;
     const float *u = st.u;
     const float *P = st.P;

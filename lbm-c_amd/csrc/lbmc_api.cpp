@@ -132,6 +132,23 @@ lbmc_status launch_step(lbmc_ctx *c, long long t, int store)
     return LBMC_OK;
 }
 
+// Resident blocks per CU of the step kernels (0: no cap), measured on
+// 256^3 D3Q15 lattices: FP32 storage 3 (BGK family) or 4 (MRT family),
+// FP16S/BF16S 5, FP64 no gain.  Only lattices well beyond the 256 MB
+// last-level cache are capped: a cache-resident lattice is latency-bound and
+// wants every wave it can get.
+int default_blocks_per_cu(const lbmc_ctx *c)
+{
+    const double pop_bytes = (double)c->sbytes * c->Q * c->npad;
+    if (pop_bytes < 512.0 * (1 << 20)) return 0;
+    switch (c->storage) {
+    case LBMC_FP32: return c->key.family == 0 ? 3 : 4;
+    case LBMC_FP16S:
+    case LBMC_BF16S: return 5;
+    default: return 0;
+    }
+}
+
 // Launch shape: one thread per node, 256-thread blocks.  With a grid that is
 // a multiple of 8 the blocks can be reordered so that each XCD works on a
 // contiguous range of nodes (LBMC_XCD_REMAP=1 turns it on).
@@ -194,6 +211,7 @@ lbmc_status run_steps(lbmc_ctx *c, long long t0, int nsteps, int store_every)
         t0++;
         nsteps--;
     }
+    auto stores = [&](long long tt) { return store_every > 0 && ((tt + 1) % store_every) == 0; };
     long long period = lcm_ll(lcm_ll(c->L[0], c->L[1]), c->L[2]);
     if (store_every > 0) period = lcm_ll(period, store_every);
     if (c->key.closure && c->closure_every > 1) period = lcm_ll(period, c->closure_every);
@@ -209,7 +227,7 @@ lbmc_status run_steps(lbmc_ctx *c, long long t0, int nsteps, int store_every)
             lbmc_status st = LBMC_OK;
             for (long long k = 0; k < period && st == LBMC_OK; k++) {
                 const long long tt = t0 + k;
-                st = launch_step(c, tt, store_every > 0 && ((tt + 1) % store_every) == 0);
+                st = launch_step(c, tt, stores(tt));
             }
             hipGraph_t g = nullptr;
             const hipError_t e = hipStreamEndCapture(c->stream, &g);
@@ -230,10 +248,8 @@ lbmc_status run_steps(lbmc_ctx *c, long long t0, int nsteps, int store_every)
             left -= period;
         }
     }
-    for (; left > 0; left--, t++) {
-        const int store = store_every > 0 && ((t + 1) % store_every) == 0;
-        if (lbmc_status st = launch_step(c, t, store)) return st;
-    }
+    for (; left > 0; left--, t++)
+        if (lbmc_status st = launch_step(c, t, stores(t))) return st;
     return LBMC_OK;
 }
 
@@ -301,6 +317,20 @@ lbmc_status lbmc_create(const lbmc_desc *desc, lbmc_ctx **out)
     if (e != hipSuccess) return bail(hip_fail(c, e, "hipSetDevice"));
     if ((e = hipStreamCreate(&c->stream)) != hipSuccess)
         return bail(hip_fail(c, e, "hipStreamCreate"));
+    {
+        // Resident blocks per CU of the step kernels.  Each resident block
+        // keeps 256 nodes x Q populations of reads and writes in flight;
+        // at full occupancy the many concurrent streams (2Q + 1 arrays)
+        // lower the HBM throughput, and a cap of a few blocks per CU
+        // (reserved LDS, which the step kernels do not use) raises it.
+        int bpc = default_blocks_per_cu(c);
+        if (const char *env = getenv("LBMC_STEP_BLOCKS_PER_CU")) bpc = atoi(env);
+        int lds_cu = 0;
+        if ((e = hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                                       desc->device)) != hipSuccess)
+            return bail(hip_fail(c, e, "device attribute"));
+        c->key.lds = (bpc >= 3 && bpc <= 16 && lds_cu > 0) ? lds_cu / bpc - 1024 : 0;
+    }
     if ((e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess)
         return bail(hip_fail(c, e, "hipEventCreate"));
     if ((e = hipMalloc(&c->f, c->sbytes * (size_t)c->Q * (size_t)c->npad)) != hipSuccess)

@@ -11,24 +11,26 @@
 
 namespace lbmc {
 
+// lds: dynamic LDS bytes per block (caps the resident blocks per CU)
 template <class L, class St, int FAM, bool GUO, bool SMAG, bool MFIX>
-static hipError_t launch_one(const StepArgs<typename St::C> &a, int grid, hipStream_t s, bool wide)
+static hipError_t launch_one(const StepArgs<typename St::C> &a, int grid, hipStream_t s, bool wide, int lds)
 {
     if (wide)
-        hipLaunchKernelGGL((step_kernel<L, St, FAM, GUO, SMAG, MFIX, true>), dim3(grid), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL((step_kernel<L, St, FAM, GUO, SMAG, MFIX, true>), dim3(grid), dim3(kBlock), lds, s, a);
     else
-        hipLaunchKernelGGL((step_kernel<L, St, FAM, GUO, SMAG, MFIX, false>), dim3(grid), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL((step_kernel<L, St, FAM, GUO, SMAG, MFIX, false>), dim3(grid), dim3(kBlock), lds, s, a);
     return hipGetLastError();
 }
 
 // closure calls: tmode 0 evaluate, 1 evaluate and record tau, 2 reuse the
 // record (grid: the persistent closure grid for 0/1, one block per 256 nodes
-// for 2)
+// for 2; lds: the step kernels' residency cap, applied to the reuse kernel,
+// which has their memory pattern)
 template <class L, class St, bool GUO, bool W>
-static hipError_t launch_closure_w(const StepArgs<float> &a, int grid, hipStream_t s)
+static hipError_t launch_closure_w(const StepArgs<float> &a, int grid, hipStream_t s, int lds)
 {
     if (a.tmode == 2)
-        hipLaunchKernelGGL((closure_reuse_kernel<L, St, GUO, W>), dim3(grid), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL((closure_reuse_kernel<L, St, GUO, W>), dim3(grid), dim3(kBlock), lds, s, a);
     else if (a.tmode == 1)
         hipLaunchKernelGGL((closure_step_kernel<L, St, GUO, true, W>), dim3(grid), dim3(kBlock), 0, s, a);
     else
@@ -37,9 +39,10 @@ static hipError_t launch_closure_w(const StepArgs<float> &a, int grid, hipStream
 }
 
 template <class L, class St, bool GUO>
-static hipError_t launch_closure(const StepArgs<float> &a, int grid, hipStream_t s, bool wide)
+static hipError_t launch_closure(const StepArgs<float> &a, int grid, hipStream_t s, bool wide, int lds)
 {
-    return wide ? launch_closure_w<L, St, GUO, true>(a, grid, s) : launch_closure_w<L, St, GUO, false>(a, grid, s);
+    return wide ? launch_closure_w<L, St, GUO, true>(a, grid, s, lds)
+                : launch_closure_w<L, St, GUO, false>(a, grid, s, lds);
 }
 
 template <class L, class St>
@@ -48,26 +51,26 @@ static hipError_t launch_lattice(const KernelKey &k, const StepArgs<typename St:
 {
     if constexpr (std::is_same<typename St::C, float>::value) {
         if (k.closure && k.family == 0) {
-            return k.guo ? launch_closure<L, St, true>(a, grid, s, k.wide != 0)
-                         : launch_closure<L, St, false>(a, grid, s, k.wide != 0);
+            return k.guo ? launch_closure<L, St, true>(a, grid, s, k.wide != 0, k.lds)
+                         : launch_closure<L, St, false>(a, grid, s, k.wide != 0, k.lds);
         }
     }
     if (k.family == 0) {
         if (k.guo) {
-            return k.smag ? launch_one<L, St, 0, true, true, false>(a, grid, s, k.wide != 0)
-                          : launch_one<L, St, 0, true, false, false>(a, grid, s, k.wide != 0);
+            return k.smag ? launch_one<L, St, 0, true, true, false>(a, grid, s, k.wide != 0, k.lds)
+                          : launch_one<L, St, 0, true, false, false>(a, grid, s, k.wide != 0, k.lds);
         }
-        return k.smag ? launch_one<L, St, 0, false, true, false>(a, grid, s, k.wide != 0)
-                      : launch_one<L, St, 0, false, false, false>(a, grid, s, k.wide != 0);
+        return k.smag ? launch_one<L, St, 0, false, true, false>(a, grid, s, k.wide != 0, k.lds)
+                      : launch_one<L, St, 0, false, false, false>(a, grid, s, k.wide != 0, k.lds);
     }
     if constexpr (L::D == 3) {
         if (k.mfix) {
-            return k.guo ? launch_one<L, St, 1, true, false, true>(a, grid, s, k.wide != 0)
-                         : launch_one<L, St, 1, false, false, true>(a, grid, s, k.wide != 0);
+            return k.guo ? launch_one<L, St, 1, true, false, true>(a, grid, s, k.wide != 0, k.lds)
+                         : launch_one<L, St, 1, false, false, true>(a, grid, s, k.wide != 0, k.lds);
         }
     }
-    return k.guo ? launch_one<L, St, 1, true, false, false>(a, grid, s, k.wide != 0)
-                 : launch_one<L, St, 1, false, false, false>(a, grid, s, k.wide != 0);
+    return k.guo ? launch_one<L, St, 1, true, false, false>(a, grid, s, k.wide != 0, k.lds)
+                 : launch_one<L, St, 1, false, false, false>(a, grid, s, k.wide != 0, k.lds);
 }
 
 template <class St>
