@@ -192,6 +192,7 @@ def main():
             lats.append(lt)
         b = lbmc.Batch(lats)
         grid, _ = b.autotune(0, steps_per_trial=20)
+        _, chunk_nodes, bpc = b.shape()
         esteps = max(64, args.steps)
         b.time_steps(1000, 64, 0)
         ems = b.time_steps(1064, esteps, 0)
@@ -199,7 +200,7 @@ def main():
         for lt in lats:
             lt.close()
         ensemble_line = {"members": 64, "nodes": EN, "MLUPS": round(EN * esteps / (ems * 1e-3) / 1e6, 1),
-                         "ms_per_step": round(ems / esteps, 5), "grid": grid,
+                         "ms_per_step": round(ems / esteps, 5), "grid": grid, "chunk_nodes": chunk_nodes, "blocks_per_cu": bpc,
                          "bytes_per_node": 2 * 9 * 4 + 1,
                          "achieved_GBs": round(73 * EN / (ems / esteps * 1e-3) / 1e9, 1)}
         log(f"bench: ensemble {ensemble_line}")

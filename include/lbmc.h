@@ -257,13 +257,21 @@ lbmc_status lbmc_batch_run(lbmc_batch *b, long long t0, int nsteps, int store_ev
 /* The same, timed with HIP events on the batch's stream (ms total). */
 lbmc_status lbmc_batch_time_steps(lbmc_batch *b, long long t0, int nsteps, int store_every, float *ms);
 
-/* Times the persistent kernel's candidate grids (1, 2, 4, 8, 16 blocks per
- * CU and one block per chunk) over steps_per_trial calls each, starting at
- * call t0 (the members advance 2 * steps_per_trial calls per candidate),
- * and keeps the fastest. */
+/* Times candidate launch shapes of the batch step kernel over
+ * steps_per_trial calls each, starting at call t0 (the members advance
+ * 2 * steps_per_trial calls per candidate), and keeps the fastest: chunks
+ * of 256, 512, 1024 or 2048 nodes with one block per chunk and the resident
+ * blocks per CU uncapped or capped at 4 or 6, and persistent grids of 2, 4,
+ * 8, 16 blocks per CU over 256-node chunks. */
 lbmc_status lbmc_batch_autotune(lbmc_batch *b, long long t0, int steps_per_trial, int *grid, float *ms_per_step);
-lbmc_status lbmc_batch_set_grid(lbmc_batch *b, int grid);
-lbmc_status lbmc_batch_info(const lbmc_batch *b, long long *nodes, int *grid, int *chunks);
+/* The batch step's launch shape.  grid: blocks (at most one per chunk; a
+ * smaller grid strides over the chunks); chunk_nodes: nodes of one member
+ * that a block takes at a time, a multiple of 256 up to 16384;
+ * blocks_per_cu: cap on the blocks resident per CU, 3..16, or 0 for none
+ * (ignored when members use the learned closure). */
+lbmc_status lbmc_batch_set_shape(lbmc_batch *b, int grid, int chunk_nodes, int blocks_per_cu);
+lbmc_status lbmc_batch_info(const lbmc_batch *b, long long *nodes, int *grid, int *chunks, int *chunk_nodes,
+                            int *blocks_per_cu);
 
 #ifdef __cplusplus
 }

@@ -44,6 +44,10 @@ struct lbmc_batch {
     const void *mlp = nullptr;  // closure weights shared by the closure members (a member's image)
     long long nodes = 0;
     int grid = 1;
+    int sub = 1;            // 256-node tiles per chunk
+    int blocks_per_cu = 0;  // residency cap of the step kernel (0: none)
+    int lds = 0;            //   as dynamic LDS bytes per block
+    int lds_cu = 0;         // LDS bytes per CU
     long long t_next = -1;  // the call the device arguments describe
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -115,11 +119,12 @@ lbmc_status upload_members(lbmc_batch *b, long long t0, int store_every)
 lbmc_status launch_one_step(lbmc_batch *b)
 {
     hipError_t e;
+    const BatchShape sh{b->grid, b->sub, b->lds};
     switch (b->storage) {
-    case LBMC_FP64: e = launch_batch_fp64(b->lattice, b->dmembers, b->chunks, b->nchunks, b->grid, b->stream); break;
-    case LBMC_FP32: e = launch_batch_st<StFP32>(b->lattice, b->dmembers, b->chunks, b->nchunks, b->grid, b->mlp, b->stream); break;
-    case LBMC_FP16S: e = launch_batch_st<StFP16S>(b->lattice, b->dmembers, b->chunks, b->nchunks, b->grid, b->mlp, b->stream); break;
-    default: e = launch_batch_st<StBF16S>(b->lattice, b->dmembers, b->chunks, b->nchunks, b->grid, b->mlp, b->stream); break;
+    case LBMC_FP64: e = launch_batch_fp64(b->lattice, b->dmembers, b->chunks, b->nchunks, b->grid, b->sub, b->lds, b->stream); break;
+    case LBMC_FP32: e = launch_batch_st<StFP32>(b->lattice, b->dmembers, b->chunks, b->nchunks, sh, b->mlp, b->stream); break;
+    case LBMC_FP16S: e = launch_batch_st<StFP16S>(b->lattice, b->dmembers, b->chunks, b->nchunks, sh, b->mlp, b->stream); break;
+    default: e = launch_batch_st<StBF16S>(b->lattice, b->dmembers, b->chunks, b->nchunks, sh, b->mlp, b->stream); break;
     }
     if (e != hipSuccess) return bfail(b, LBMC_ERR_DEVICE, "batch step launch: %s", hipGetErrorString(e));
     const int nb = (b->count + 63) / 64;
@@ -180,6 +185,24 @@ lbmc_status batch_steps(lbmc_batch *b, long long t0, int nsteps)
         if (st != LBMC_OK) return st;
     }
     b->t_next = t0 + nsteps;
+    return LBMC_OK;
+}
+
+// Chunk list for `sub` tiles per chunk: (member, first node) of every chunk,
+// member by member.
+lbmc_status build_chunks(lbmc_batch *b, int sub)
+{
+    const long long cn = (long long)sub * kBlock;
+    std::vector<int2> ch;
+    for (int k = 0; k < b->count; k++)
+        for (long long n = 0; n < b->members[(size_t)k]->N; n += cn) ch.push_back(make_int2(k, (int)n));
+    if (b->chunks) B_HIP(b, hipFree(b->chunks));
+    b->chunks = nullptr;
+    B_HIP(b, hipMalloc((void **)&b->chunks, sizeof(int2) * ch.size()));
+    B_HIP(b, hipMemcpy(b->chunks, ch.data(), sizeof(int2) * ch.size(), hipMemcpyHostToDevice));
+    b->nchunks = (int)ch.size();
+    b->sub = sub;
+    b->grid = std::min(b->grid, b->nchunks);
     return LBMC_OK;
 }
 
@@ -250,25 +273,23 @@ lbmc_status lbmc_batch_create(lbmc_ctx *const *members, int count, lbmc_batch **
         lbmc_batch_destroy(b);
         return st;
     };
-    std::vector<int2> ch;
-    ch.reserve((size_t)nchunks);
-    for (int k = 0; k < count; k++)
-        for (long long n = 0; n < members[k]->N; n += kBlock) ch.push_back(make_int2(k, (int)n));
     hipError_t e = hipSetDevice(b->device);
     if (e == hipSuccess) e = hipStreamCreate(&b->stream);
     if (e == hipSuccess) e = hipEventCreate(&b->ev0);
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
     const size_t msz = b->cdouble ? sizeof(BatchMember<double>) : sizeof(BatchMember<float>);
     if (e == hipSuccess) e = hipMalloc(&b->dmembers, msz * (size_t)count);
-    if (e == hipSuccess) e = hipMalloc((void **)&b->chunks, sizeof(int2) * ch.size());
-    if (e == hipSuccess) e = hipMemcpy(b->chunks, ch.data(), sizeof(int2) * ch.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) return bail(bfail(b, LBMC_ERR_DEVICE, "lbmc_batch_create: %s", hipGetErrorString(e)));
+    b->grid = b->nchunks;
+    if (lbmc_status st = build_chunks(b, 1)) return bail(st);
     // default persistent grid: 8 blocks per CU, at most one block per chunk
     int ncu = 256;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, b->device) == hipSuccess && prop.multiProcessorCount > 0)
         ncu = prop.multiProcessorCount;
     b->grid = (int)std::min<long long>(nchunks, 8LL * ncu);
+    if (hipDeviceGetAttribute(&b->lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, b->device) != hipSuccess)
+        b->lds_cu = 0;
     *out = b;
     return LBMC_OK;
 }
@@ -316,9 +337,20 @@ lbmc_status lbmc_batch_time_steps(lbmc_batch *b, long long t0, int nsteps, int s
     return LBMC_OK;
 }
 
-// Times candidate persistent grids (blocks per CU 1..16, and one block per
-// chunk) over a few steps from the current state and keeps the fastest.
-// The steps advance the members' state; callers tune before the timed run.
+// Residency cap of the batch step kernel: at most `per` blocks per CU,
+// through dynamic LDS that the kernel does not use (0: no cap).  Not with
+// closure members, whose kernel stages the network in static LDS.
+void set_blocks_per_cu(lbmc_batch *b, int per)
+{
+    b->blocks_per_cu = (per >= 3 && per <= 16 && b->lds_cu > 0 && !b->mlp) ? per : 0;
+    b->lds = b->blocks_per_cu ? b->lds_cu / b->blocks_per_cu - 1024 : 0;
+}
+
+// Times candidate launch shapes over a few steps from the current state and
+// keeps the fastest: chunks of 1, 2, 4 or 8 tiles with one block per chunk
+// and the resident blocks per CU uncapped or capped at 4 or 6, and
+// persistent grids of 2, 4, 8, 16 blocks per CU over one-tile chunks.  The
+// steps advance the members' state; callers tune before the timed run.
 lbmc_status lbmc_batch_autotune(lbmc_batch *b, long long t0, int steps_per_trial, int *grid, float *ms_per_step)
 {
     if (!b || steps_per_trial <= 0 || t0 < 0) return bfail(b, LBMC_ERR_INVALID, "lbmc_batch_autotune: bad argument");
@@ -327,17 +359,24 @@ lbmc_status lbmc_batch_autotune(lbmc_batch *b, long long t0, int steps_per_trial
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, b->device) == hipSuccess && prop.multiProcessorCount > 0)
         ncu = prop.multiProcessorCount;
-    std::vector<int> cand;
-    for (int per : {1, 2, 4, 8, 16}) cand.push_back((int)std::min<long long>(b->nchunks, (long long)per * ncu));
-    cand.push_back(b->nchunks);
-    std::sort(cand.begin(), cand.end());
-    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    struct Cand {
+        int sub, grid_per_cu, cap;  // grid_per_cu 0: one block per chunk
+    };
+    std::vector<Cand> cand;
+    for (int sub : {1, 2, 4, 8})
+        for (int cap : {0, 4, 6}) cand.push_back({sub, 0, cap});
+    for (int per : {2, 4, 8, 16}) cand.push_back({1, per, 0});
     float best = -1.f;
+    Cand best_c{b->sub, 0, b->blocks_per_cu};
     int best_grid = b->grid;
     long long t = t0;
-    for (int g : cand) {
+    for (const Cand &cd : cand) {
+        set_blocks_per_cu(b, cd.cap);
+        if (cd.cap && !b->blocks_per_cu) continue;  // cap not available
         drop_batch_graph(b);
-        b->grid = g;
+        if (cd.sub != b->sub)
+            if (lbmc_status st = build_chunks(b, cd.sub)) return st;
+        b->grid = cd.grid_per_cu ? (int)std::min<long long>(b->nchunks, (long long)cd.grid_per_cu * ncu) : b->nchunks;
         float ms = 0.f;
         // one untimed pass (graph build), then the timed one
         lbmc_status st = lbmc_batch_time_steps(b, t, steps_per_trial, 0, &ms);
@@ -348,30 +387,45 @@ lbmc_status lbmc_batch_autotune(lbmc_batch *b, long long t0, int steps_per_trial
         t += steps_per_trial;
         if (best < 0.f || ms < best) {
             best = ms;
-            best_grid = g;
+            best_c = cd;
+            best_c.cap = b->blocks_per_cu;
+            best_grid = b->grid;
         }
     }
     drop_batch_graph(b);
-    b->grid = best_grid;
-    if (grid) *grid = best_grid;
+    if (best_c.sub != b->sub)
+        if (lbmc_status st = build_chunks(b, best_c.sub)) return st;
+    b->grid = std::min(best_grid, b->nchunks);
+    set_blocks_per_cu(b, best_c.cap);
+    if (grid) *grid = b->grid;
     if (ms_per_step) *ms_per_step = best / (float)steps_per_trial;
     return LBMC_OK;
 }
 
-lbmc_status lbmc_batch_info(const lbmc_batch *b, long long *nodes, int *grid, int *chunks)
+lbmc_status lbmc_batch_info(const lbmc_batch *b, long long *nodes, int *grid, int *chunks, int *chunk_nodes,
+                            int *blocks_per_cu)
 {
     if (!b) return LBMC_ERR_INVALID;
     if (nodes) *nodes = b->nodes;
     if (grid) *grid = b->grid;
     if (chunks) *chunks = b->nchunks;
+    if (chunk_nodes) *chunk_nodes = b->sub * kBlock;
+    if (blocks_per_cu) *blocks_per_cu = b->blocks_per_cu;
     return LBMC_OK;
 }
 
-lbmc_status lbmc_batch_set_grid(lbmc_batch *b, int grid)
+lbmc_status lbmc_batch_set_shape(lbmc_batch *b, int grid, int chunk_nodes, int blocks_per_cu)
 {
-    if (!b || grid <= 0) return bfail(b, LBMC_ERR_INVALID, "lbmc_batch_set_grid: grid must be positive");
+    if (!b || grid <= 0 || blocks_per_cu < 0 || chunk_nodes < kBlock || chunk_nodes % kBlock ||
+        chunk_nodes > 64 * kBlock)
+        return bfail(b, LBMC_ERR_INVALID,
+                     "lbmc_batch_set_shape: grid > 0, chunk_nodes a multiple of 256 up to 16384, blocks_per_cu >= 0");
+    B_HIP(b, hipSetDevice(b->device));
     drop_batch_graph(b);
+    if (chunk_nodes / kBlock != b->sub)
+        if (lbmc_status st = build_chunks(b, chunk_nodes / kBlock)) return st;
     b->grid = std::min(grid, b->nchunks);
+    set_blocks_per_cu(b, blocks_per_cu);
     return LBMC_OK;
 }
 

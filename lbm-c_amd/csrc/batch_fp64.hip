@@ -5,10 +5,10 @@
 
 namespace lbmc {
 
-hipError_t launch_batch_fp64(int lattice, const void *members, const int2 *chunks, int nchunks, int grid,
-                             hipStream_t s)
+hipError_t launch_batch_fp64(int lattice, const void *members, const int2 *chunks, int nchunks, int grid, int sub,
+                             int lds, hipStream_t s)
 {
-    return launch_batch_st<StFP64>(lattice, members, chunks, nchunks, grid, nullptr, s);
+    return launch_batch_st<StFP64>(lattice, members, chunks, nchunks, BatchShape{grid, sub, lds}, nullptr, s);
 }
 
 }  // namespace lbmc

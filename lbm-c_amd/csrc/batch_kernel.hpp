@@ -45,12 +45,19 @@ __device__ __forceinline__ void batch_node(const StepArgs<typename St::C> &a, in
     }
 }
 
+// A chunk is `sub` consecutive 256-node tiles of one member, starting at
+// node ch.y: the block reads the chunk entry and the member's arguments once
+// (a chain of dependent loads) and then walks its tiles.  members and
+// chunks are read-only for the kernel's lifetime and reached through no
+// other pointer (__restrict__), so their loads are scalar and stay in scalar
+// registers across the population stores.
 // CLOSURE: some members use the learned closure (one shared network, staged
-// into LDS once per block); their chunks run the network over the whole
+// into LDS once per block); their tiles run the network over the whole
 // wave, lanes past the member's last node evaluating a copy of it.
 template <class L, class St, bool CLOSURE>
-__global__ __launch_bounds__(kBlock) void batch_step_kernel(const BatchMember<typename St::C> *members,
-                                                            const int2 *chunks, int nchunks, const void *mlp)
+__global__ __launch_bounds__(kBlock) void batch_step_kernel(const BatchMember<typename St::C> *__restrict__ members,
+                                                            const int2 *__restrict__ chunks, int nchunks,
+                                                            const void *__restrict__ mlp, int sub)
 {
     using C = typename St::C;
     constexpr bool CL = CLOSURE && std::is_same<C, float>::value;
@@ -67,25 +74,29 @@ __global__ __launch_bounds__(kBlock) void batch_step_kernel(const BatchMember<ty
         const BatchMember<C> &m = members[ch.x];
         const StepArgs<C> &a = m.a;
         const int variant = m.variant;
-        const int n0 = ch.y + (int)threadIdx.x;
-        if constexpr (CL) {
-            if (variant >= 8) {
-                const bool live = n0 < a.nnodes;
-                const int n = live ? n0 : a.nnodes - 1;
-                int dd[3][3];
-                slot_offsets<L>(a, n, dd);
-                float v[L::Q];
-                gather_node<L, St, false>(a, n, dd, v);
-                if (variant == 9) closure_update<L, St, true, false>(a, n, live, v, w, xb + (threadIdx.x & ~63u));
-                else closure_update<L, St, false, false>(a, n, live, v, w, xb + (threadIdx.x & ~63u));
-                if (live) scatter_node<L, St, false>(a, n, dd, v);
-                continue;
+        for (int j = 0; j < sub; j++) {
+            const int t0 = ch.y + j * kBlock;  // first node of the tile
+            if (t0 >= a.nnodes) break;         // block-uniform
+            const int n0 = t0 + (int)threadIdx.x;
+            if constexpr (CL) {
+                if (variant >= 8) {
+                    const bool live = n0 < a.nnodes;
+                    const int n = live ? n0 : a.nnodes - 1;
+                    int dd[3][3];
+                    slot_offsets<L>(a, n, dd);
+                    float v[L::Q];
+                    gather_node<L, St, false>(a, n, dd, v);
+                    if (variant == 9) closure_update<L, St, true, false>(a, n, live, v, w, xb + (threadIdx.x & ~63u));
+                    else closure_update<L, St, false, false>(a, n, live, v, w, xb + (threadIdx.x & ~63u));
+                    if (live) scatter_node<L, St, false>(a, n, dd, v);
+                    continue;
+                }
             }
-        }
-        if (n0 < a.nnodes) {
-            int dd[3][3];
-            slot_offsets<L>(a, n0, dd);
-            batch_node<L, St>(a, variant, n0, dd);
+            if (n0 < a.nnodes) {
+                int dd[3][3];
+                slot_offsets<L>(a, n0, dd);
+                batch_node<L, St>(a, variant, n0, dd);
+            }
         }
     }
 }
@@ -118,26 +129,34 @@ __global__ void batch_advance_kernel(BatchMember<C> *members, int count)
     members[k] = m;
 }
 
+// Launch shape of the batch step: grid blocks, `sub` 256-node tiles per
+// chunk, and lds: dynamic LDS bytes per block (unused by the kernel; caps
+// the blocks resident per CU)
+struct BatchShape {
+    int grid, sub, lds;
+};
+
 template <class L, class St>
-hipError_t launch_batch_lattice(const void *members, const int2 *chunks, int nchunks, int grid, const void *mlp,
-                                hipStream_t s)
+hipError_t launch_batch_lattice(const void *members, const int2 *chunks, int nchunks, BatchShape sh,
+                                const void *mlp, hipStream_t s)
 {
     const auto *mm = (const BatchMember<typename St::C> *)members;
     if (mlp)
-        hipLaunchKernelGGL((batch_step_kernel<L, St, true>), dim3(grid), dim3(kBlock), 0, s, mm, chunks, nchunks, mlp);
+        hipLaunchKernelGGL((batch_step_kernel<L, St, true>), dim3(sh.grid), dim3(kBlock), sh.lds, s, mm, chunks,
+                           nchunks, mlp, sh.sub);
     else
-        hipLaunchKernelGGL((batch_step_kernel<L, St, false>), dim3(grid), dim3(kBlock), 0, s, mm, chunks, nchunks,
-                           mlp);
+        hipLaunchKernelGGL((batch_step_kernel<L, St, false>), dim3(sh.grid), dim3(kBlock), sh.lds, s, mm, chunks,
+                           nchunks, mlp, sh.sub);
     return hipGetLastError();
 }
 
 // mlp: the shared closure weights when some member uses the closure, else null
 template <class St>
-hipError_t launch_batch_st(int lattice, const void *members, const int2 *chunks, int nchunks, int grid,
+hipError_t launch_batch_st(int lattice, const void *members, const int2 *chunks, int nchunks, BatchShape sh,
                            const void *mlp, hipStream_t s)
 {
-    return lattice == 0 ? launch_batch_lattice<D2Q9, St>(members, chunks, nchunks, grid, mlp, s)
-                        : launch_batch_lattice<D3Q15, St>(members, chunks, nchunks, grid, mlp, s);
+    return lattice == 0 ? launch_batch_lattice<D2Q9, St>(members, chunks, nchunks, sh, mlp, s)
+                        : launch_batch_lattice<D3Q15, St>(members, chunks, nchunks, sh, mlp, s);
 }
 
 }  // namespace lbmc

@@ -48,8 +48,8 @@ hipError_t launch_links(int lattice, const uint8_t *mask, const uint16_t *cls, i
 hipError_t launch_copy(const void *src, void *dst, size_t nbytes, hipStream_t s);
 
 // batched step, batch_fp64.hip (the other storages are launched in batch.hip)
-hipError_t launch_batch_fp64(int lattice, const void *members, const int2 *chunks, int nchunks, int grid,
-                             hipStream_t s);
+hipError_t launch_batch_fp64(int lattice, const void *members, const int2 *chunks, int nchunks, int grid, int sub,
+                             int lds, hipStream_t s);
 
 // learned closure, mlp.hip: operand image of a model's current weights with
 // the fallback band, into device memory dst on stream s

@@ -15,6 +15,22 @@
 
 namespace lbmc {
 
+// Loads and stores through the global address space.  All lattice buffers
+// are device (global) memory; kernels that read their buffer pointers from
+// device memory (the batch) would otherwise access them as generic (flat)
+// pointers.
+template <class T>
+__device__ __forceinline__ T ldg(const T *p)
+{
+    return *(const __attribute__((address_space(1))) T *)p;
+}
+
+template <class T>
+__device__ __forceinline__ void stg(T *p, T v)
+{
+    *(__attribute__((address_space(1))) T *)p = v;
+}
+
 // 1/x: FP32 uses the hardware reciprocal (1 ulp); FP64 divides exactly
 template <class C>
 __device__ __forceinline__ C recip(C x)

@@ -47,14 +47,14 @@ __device__ __forceinline__ void node_update(const StepArgs<typename St::C> &a, i
     // FP32 compute: plain fluid nodes of the BGK family take the regrouped
     // update (fast_bgk.hpp); everything else follows the reference's order.
     constexpr bool FAST = std::is_same<C, float>::value;
-    const unsigned m = a.mask[n];
+    const unsigned m = ldg(a.mask + n);
     if constexpr (FAST && FAM == 0) {
         if (m == 0) {
             C F[3] = {C(0), C(0), C(0)};
             if constexpr (GUO) {
                 sfor<0, D>([&](auto dd) {
                     constexpr int d = decltype(dd)::value;
-                    F[d] = a.force[0] ? a.force[d][n] : a.F[d];
+                    F[d] = a.force[0] ? ldg(a.force[d] + n) : a.F[d];
                 });
             }
             C rho, u[3];
@@ -62,9 +62,9 @@ __device__ __forceinline__ void node_update(const StepArgs<typename St::C> &a, i
             if (a.store) {
                 sfor<0, D>([&](auto dd) {
                     constexpr int d = decltype(dd)::value;
-                    a.u[d][n] = u[d];
+                    stg(a.u[d] + n, u[d]);
                 });
-                a.rho[n] = rho;
+                stg(a.rho + n, rho);
             }
             return;
         }
@@ -107,7 +107,7 @@ __device__ __forceinline__ void node_update(const StepArgs<typename St::C> &a, i
     }
 
     // ---- node class
-    const uint16_t desc = m ? a.cls[m] : (uint16_t)0;
+    const uint16_t desc = m ? ldg(a.cls + m) : (uint16_t)0;
 
     bool stored_done = false;
     if (desc & CLS_SOLID) {
@@ -140,18 +140,18 @@ __device__ __forceinline__ void node_update(const StepArgs<typename St::C> &a, i
     } else {
         C B = C(0);
         if (desc) {
-            if (desc & CLS_POROUS) B = a.porosity[n];
+            if (desc & CLS_POROUS) B = ldg(a.porosity + n);
             const int mc = cls_macro(desc);
             if (mc) {
                 // presc_*, macro_bc.cu:19-125
                 const unsigned ps = macro_set(mc);
                 if (ps & 1u) {
-                    rho = a.rho[n];
+                    rho = ldg(a.rho + n);
                     srho = St::shifted ? rho - C(1) : rho;
                 }
                 sfor<0, D>([&](auto dd) {
                     constexpr int d = decltype(dd)::value;
-                    if (ps & (2u << d)) u[d] = a.u[d][n];
+                    if (ps & (2u << d)) u[d] = ldg(a.u[d] + n);
                 });
             }
             const int mi = cls_micro(desc);
@@ -166,7 +166,7 @@ __device__ __forceinline__ void node_update(const StepArgs<typename St::C> &a, i
             if (a.force[0]) {
                 sfor<0, D>([&](auto dd) {
                     constexpr int d = decltype(dd)::value;
-                    F[d] = a.force[d][n];
+                    F[d] = ldg(a.force[d] + n);
                 });
             } else {
                 sfor<0, D>([&](auto dd) {
@@ -291,9 +291,9 @@ __device__ __forceinline__ void node_update(const StepArgs<typename St::C> &a, i
     if (a.store) {
         sfor<0, D>([&](auto dd) {
             constexpr int d = decltype(dd)::value;
-            a.u[d][n] = u[d];
+            stg(a.u[d] + n, u[d]);
         });
-        a.rho[n] = rho;
+        stg(a.rho + n, rho);
     }
 }
 
@@ -309,10 +309,10 @@ __device__ __forceinline__ void gather_node(const StepArgs<typename St::C> &a, i
         constexpr int i = decltype(ii)::value;
         const int k = n + dd[0][L::ex[i] + 1] + dd[1][L::ey[i] + 1] + dd[2][L::ez[i] + 1];
         if constexpr (WIDE) {
-            v[i] = St::load(fb[(size_t)i * a.npad + k]);
+            v[i] = St::load(ldg(fb + (size_t)i * a.npad + k));
         } else {
             const char *base = (const char *)(fb + (size_t)i * a.npad);
-            v[i] = St::load(*(const S *)(base + (unsigned)k * (unsigned)sizeof(S)));
+            v[i] = St::load(ldg((const S *)(base + (unsigned)k * (unsigned)sizeof(S))));
         }
     });
 }
@@ -328,10 +328,10 @@ __device__ __forceinline__ void scatter_node(const StepArgs<typename St::C> &a, 
         constexpr int i = decltype(ii)::value;
         const int k = n + dd[0][L::ex[i] + 1] + dd[1][L::ey[i] + 1] + dd[2][L::ez[i] + 1];
         if constexpr (WIDE) {
-            fb[(size_t)i * a.npad + k] = St::store(v[i]);
+            stg(fb + (size_t)i * a.npad + k, St::store(v[i]));
         } else {
             char *base = (char *)(fb + (size_t)i * a.npad);
-            *(S *)(base + (unsigned)k * (unsigned)sizeof(S)) = St::store(v[i]);
+            stg((S *)(base + (unsigned)k * (unsigned)sizeof(S)), St::store(v[i]));
         }
     });
 }

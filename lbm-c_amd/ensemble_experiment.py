@@ -68,11 +68,12 @@ def main():
         lats = build(specs, st)
         b = lbmc.Batch(lats)
         grid, tune_ms = b.autotune(0, steps_per_trial=20)
+        shape = b.shape()
         b.close()
         for lat in lats:
             lat.init_static()
         b = lbmc.Batch(lats)
-        b.set_grid(grid)
+        b.set_shape(*shape)
         b.time_steps(0, 64, 0)  # warm-up (graph build)
         ms = b.time_steps(64, args.time_steps, 0)
         info = b.info()
@@ -80,7 +81,8 @@ def main():
         for lat in lats:
             lat.close()
         thr[name] = {"MLUPS": round(N * args.time_steps / (ms * 1e-3) / 1e6, 1),
-                     "ms_per_step": round(ms / args.time_steps, 5), "grid": grid, "chunks": info["chunks"]}
+                     "ms_per_step": round(ms / args.time_steps, 5), "grid": grid, "chunks": info["chunks"],
+                     "chunk_nodes": info["chunk_nodes"], "blocks_per_cu": info["blocks_per_cu"]}
         log(f"{name}: {thr[name]}")
     res["throughput"] = thr
 

@@ -40,7 +40,7 @@ EXPORTS = [
     "lbmc_mlp_get_params", "lbmc_mlp_set_scaling", "lbmc_mlp_forward", "lbmc_mlp_loss_grad",
     "lbmc_mlp_set_data", "lbmc_mlp_train", "lbmc_set_closure", "lbmc_set_closure_interval",
     "lbmc_batch_create", "lbmc_batch_destroy", "lbmc_batch_last_error", "lbmc_batch_run",
-    "lbmc_batch_time_steps", "lbmc_batch_autotune", "lbmc_batch_set_grid", "lbmc_batch_info",
+    "lbmc_batch_time_steps", "lbmc_batch_autotune", "lbmc_batch_set_shape", "lbmc_batch_info",
 ]
 
 MLP_NPARAM = 4737
@@ -127,9 +127,9 @@ def _load():
         "lbmc_batch_run": (ct.c_int, [P, ct.c_longlong, ct.c_int, ct.c_int]),
         "lbmc_batch_time_steps": (ct.c_int, [P, ct.c_longlong, ct.c_int, ct.c_int, fp]),
         "lbmc_batch_autotune": (ct.c_int, [P, ct.c_longlong, ct.c_int, ct.POINTER(ct.c_int), fp]),
-        "lbmc_batch_set_grid": (ct.c_int, [P, ct.c_int]),
+        "lbmc_batch_set_shape": (ct.c_int, [P, ct.c_int, ct.c_int, ct.c_int]),
         "lbmc_batch_info": (ct.c_int, [P, ct.POINTER(ct.c_longlong), ct.POINTER(ct.c_int),
-                                       ct.POINTER(ct.c_int)]),
+                                       ct.POINTER(ct.c_int), ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -484,12 +484,21 @@ class Batch:
                                              ct.byref(ms)))
         return g.value, float(ms.value)
 
-    def set_grid(self, grid):
-        self._check(_lib.lbmc_batch_set_grid(self._h, int(grid)))
+    def set_shape(self, grid, chunk_nodes=256, blocks_per_cu=0):
+        self._check(_lib.lbmc_batch_set_shape(self._h, int(grid), int(chunk_nodes), int(blocks_per_cu)))
 
     def info(self):
         n = ct.c_longlong()
         g = ct.c_int()
         c = ct.c_int()
-        self._check(_lib.lbmc_batch_info(self._h, ct.byref(n), ct.byref(g), ct.byref(c)))
-        return {"nodes": n.value, "grid": g.value, "chunks": c.value}
+        cn = ct.c_int()
+        bpc = ct.c_int()
+        self._check(_lib.lbmc_batch_info(self._h, ct.byref(n), ct.byref(g), ct.byref(c), ct.byref(cn),
+                                         ct.byref(bpc)))
+        return {"nodes": n.value, "grid": g.value, "chunks": c.value, "chunk_nodes": cn.value,
+                "blocks_per_cu": bpc.value}
+
+    def shape(self):
+        """(grid, chunk_nodes, blocks_per_cu): the launch shape, e.g. after autotune."""
+        i = self.info()
+        return i["grid"], i["chunk_nodes"], i["blocks_per_cu"]

@@ -23,11 +23,11 @@ def _members(lattice):
     return out
 
 
-def _run_batch(group, storage, t0, steps, store_every=1, grid=None):
+def _run_batch(group, storage, t0, steps, store_every=1, shape=None):
     lats = [cases.build(lbmc.Lattice, c, storage=storage) for c in group]
     b = lbmc.Batch(lats)
-    if grid is not None:
-        b.set_grid(grid)
+    if shape is not None:
+        b.set_shape(*shape)
     b.run(t0, steps, store_every)
     out = [(lat.get_f(), lat.get_macros()) for lat in lats]
     b.close()
@@ -93,12 +93,27 @@ def test_autotune_then_run_matches_oracle():
     b = lbmc.Batch(lats)
     info = b.info()
     grid, ms = b.autotune(0, steps_per_trial=8)
-    assert 1 <= grid <= info["chunks"] and ms > 0
+    shape = b.shape()
+    assert shape[0] == grid and ms > 0
+    assert 1 <= grid <= b.info()["chunks"] <= info["chunks"]
+    assert shape[1] in (256, 512, 1024, 2048) and shape[2] in (0, 4, 6)
     b.close()
     for lat in lats:
         lat.close()
-    got = _run_batch(group, lbmc.FP64, 0, 40, grid=grid)
+    got = _run_batch(group, lbmc.FP64, 0, 40, shape=shape)
     ref = _oracle(group, 0, 40)
+    for c, (fg, _), (fr, _) in zip(group, got, ref):
+        assert np.array_equal(fg, fr), c["name"]
+
+
+@pytest.mark.parametrize("shape", [(1 << 20, 256, 3), (1 << 20, 1024, 5), (7, 512, 0), (1 << 20, 2048, 0)],
+                         ids=["cap3", "chunk1024_cap5", "grid7_chunk512", "chunk2048"])
+def test_launch_shapes_match_oracle(shape):
+    """Residency caps, multi-tile chunks (members smaller than a chunk, and
+    chunks ending past a member's last node) and a small strided grid."""
+    group = _members(lbmc.D3Q15)
+    got = _run_batch(group, lbmc.FP64, 3, 37, shape=shape)
+    ref = _oracle(group, 3, 37)
     for c, (fg, _), (fr, _) in zip(group, got, ref):
         assert np.array_equal(fg, fr), c["name"]
 
