@@ -90,6 +90,7 @@ def main():
         sys.exit(2)
 
     import numpy as np
+    import closure
     import lbmc
     import workloads
 
@@ -153,8 +154,6 @@ def main():
         # same lattice and state, relaxation time from the closure network
         # (He-initialised weights; the cost does not depend on their values
         # beyond the share of nodes that fall back to Smagorinsky)
-        import closure
-
         mu = np.zeros(7, dtype=np.float32)
         mu[6] = 1.0
         isd = np.full(7, 1e3, dtype=np.float32)
@@ -177,32 +176,53 @@ def main():
 
     ensemble_line = None
     if not args.no_ensemble:
-        # 64 uneven D2Q9 channels with discs (seeded), FP32, one persistent
-        # batched launch per step, autotuned grid
+        # 64 uneven D2Q9 channels with discs (seeded), FP32, one batched
+        # launch per step, autotuned launch shape.  Members are MRT or
+        # BGK + learned closure (one shared network, He-initialised, centred
+        # on each member's own tau by its shift; fallback band [0.5005, 2]);
+        # timed with the network evaluated every call and every 8th call, and
+        # with plain BGK in place of BGK + learned.
         specs = workloads.ensemble_d2q9(64, 0)
         EN = sum(m["length"][0] * m["length"][1] for m in specs)
-        lats = []
-        for m in specs:
-            Lx, Ly = m["length"]
-            lt = lbmc.Lattice(lbmc.D2Q9, (Lx, Ly), lbmc.MRT if m["collision"] == "MRT" else lbmc.BGK,
-                              forcing=True, tau=m["tau"], tau_mrt=m["rates"], storage=lbmc.FP32)
-            lt.set_fields(geometry=m["geometry"].reshape(-1),
-                          force=[np.full(Lx * Ly, m["force"]), np.zeros(Lx * Ly)])
-            lt.init_static()
-            lats.append(lt)
-        b = lbmc.Batch(lats)
-        grid, _ = b.autotune(0, steps_per_trial=20)
-        _, chunk_nodes, bpc = b.shape()
-        esteps = max(64, args.steps)
-        b.time_steps(1000, 64, 0)
-        ems = b.time_steps(1064, esteps, 0)
-        b.close()
-        for lt in lats:
-            lt.close()
-        ensemble_line = {"members": 64, "nodes": EN, "MLUPS": round(EN * esteps / (ems * 1e-3) / 1e6, 1),
-                         "ms_per_step": round(ems / esteps, 5), "grid": grid, "chunk_nodes": chunk_nodes, "blocks_per_cu": bpc,
-                         "bytes_per_node": 2 * 9 * 4 + 1,
-                         "achieved_GBs": round(73 * EN / (ems / esteps * 1e-3) / 1e9, 1)}
+        # input scaling for channel flows (Pi / rho ~ 1e-4), output tau' =
+        # 0.55 + 0.01 y before the member's shift
+        ENS_TAU_REF = 0.55
+        ens_mu = np.array([0, 0, 0, 0, 0, 0, 1.0], dtype=np.float32)
+        ens_isd = np.array([1e4] * 6 + [100.0], dtype=np.float32)
+        model = lbmc.Mlp(closure.init_params(0), mu=ens_mu, isd=ens_isd, out_a=0.01, out_b=ENS_TAU_REF)
+
+        def run_ensemble(every):
+            lats = []
+            for m in specs:
+                Lx, Ly = m["length"]
+                lt = lbmc.Lattice(lbmc.D2Q9, (Lx, Ly), lbmc.MRT if m["collision"] == "MRT" else lbmc.BGK,
+                                  forcing=True, tau=m["tau"], tau_mrt=m["rates"], storage=lbmc.FP32)
+                lt.set_fields(geometry=m["geometry"].reshape(-1),
+                              force=[np.full(Lx * Ly, m["force"]), np.zeros(Lx * Ly)])
+                lt.init_static()
+                if every and m["collision"] != "MRT":
+                    lt.set_closure(model, 0.5005, 2.0)
+                    lt.set_closure_shift(m["tau"] - ENS_TAU_REF)
+                    lt.set_closure_interval(every)
+                lats.append(lt)
+            b = lbmc.Batch(lats)
+            grid, _ = b.autotune(0, steps_per_trial=16)
+            _, chunk_nodes, bpc = b.shape()
+            esteps = max(64, args.steps)
+            b.time_steps(1000, 64, 0)
+            ems = b.time_steps(1064, esteps, 0)
+            b.close()
+            for lt in lats:
+                lt.close()
+            return {"MLUPS": round(EN * esteps / (ems * 1e-3) / 1e6, 1), "ms_per_step": round(ems / esteps, 5),
+                    "grid": grid, "chunk_nodes": chunk_nodes, "blocks_per_cu": bpc,
+                    "achieved_GBs": round(73 * EN / (ems / esteps * 1e-3) / 1e9, 1)}
+
+        ensemble_line = {"members": 64, "nodes": EN, "bytes_per_node": 2 * 9 * 4 + 1,
+                         "members_kind": "MRT or BGK + learned closure",
+                         "closure_every_1": run_ensemble(1), "closure_every_8": run_ensemble(8),
+                         "plain_bgk_mrt": run_ensemble(0)}
+        model.close()
         log(f"bench: ensemble {ensemble_line}")
 
     cpu = None

@@ -231,6 +231,12 @@ lbmc_status lbmc_set_closure(lbmc_ctx *ctx, const lbmc_mlp *model, double tau_lo
  * tau of the current state).  every = 1 (the default) evaluates every call. */
 lbmc_status lbmc_set_closure_interval(lbmc_ctx *ctx, int every);
 
+/* Shift dtau added to the network's tau' on this lattice before the band
+ * check: tau' = out_a * y + out_b + dtau (0 by default).  Lets one model,
+ * trained around a base tau, serve lattices of other base taus (batch
+ * members with their own tau, for instance). */
+lbmc_status lbmc_set_closure_shift(lbmc_ctx *ctx, double dtau);
+
 /* ---- Batches of lattices ---------------------------------------------------
  *
  * Steps an ensemble of independent lattices of different sizes together:
@@ -240,9 +246,10 @@ lbmc_status lbmc_set_closure_interval(lbmc_ctx *ctx, int every);
  * an addition.  Members are contexts made by lbmc_create and set up as usual
  * (fields and initial state); they share lattice type, storage and device,
  * may differ in size, collision type, forcing, tau and boundary codes, and
- * may not use sf boundaries or the learned closure.  Members must outlive
- * the batch; their state is read at every lbmc_batch_run, and lbmc_get_* on
- * a member reads what the batch computed.
+ * may not use sf boundaries.  Members with the learned closure share one
+ * model and band; each keeps its own evaluation interval, on the schedule
+ * of lbmc_run.  Members must outlive the batch; their state is read at every
+ * lbmc_batch_run, and lbmc_get_* on a member reads what the batch computed.
  */
 typedef struct lbmc_batch lbmc_batch;
 

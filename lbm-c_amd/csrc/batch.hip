@@ -1,21 +1,27 @@
 // Batched stepping of several lattices (ensembles of uneven lattices): one
-// persistent launch per time step covers the nodes of every member.
+// call for every member is one pass of launches.
 //
 // The members are ordinary contexts (lbmc_create + lbmc_set_fields + an
 // initial condition).  The batch keeps, in device memory, each member's
-// kernel arguments (its buffers, its collision variant and the slot offsets
-// of the current call t) and a list of 256-node chunks over all members.
-// A step is two launches:
-//   batch_step_kernel     blocks stride over the chunk list; a chunk belongs
-//                         to one member, so the member's arguments and its
-//                         collision variant are uniform over the block;
-//   batch_advance_kernel  one thread per member moves its arguments from
-//                         call t to t + 1 (shift counters, slot offsets and
-//                         the macro-store flag), so the kernels never take t
-//                         as an argument.
-// Because nothing in the launches depends on t, a fixed run of steps is
-// captured once into a HIP graph and replayed for any t.  The grid of the
-// persistent kernel is chosen by lbmc_batch_autotune (timed candidates).
+// kernel arguments (its buffers, its collision variant, the slot offsets of
+// the current call t and its closure schedule) and lists of chunks (runs of
+// 256-node tiles of one member) over all members and over the closure
+// members.  A call is
+//   batch_step_kernel     blocks take chunks (striding over the list when
+//                         the grid is smaller); a chunk belongs to one
+//                         member, so its arguments and collision variant
+//                         are uniform over the block; chunks of closure
+//                         members that evaluate the network are skipped;
+//   batch_eval_kernel     (closure members only) those chunks, with the
+//                         network; returns at once when no member evaluates;
+//   batch_advance_kernel  one block moves every member's arguments from
+//                         call t to t + 1 (shift counters, slot offsets, the
+//                         macro-store flag, the closure phase) and sets the
+//                         device flag of the evaluation kernel, so no kernel
+//                         takes t as an argument.
+// Because nothing in the launches depends on t, a fixed run of calls is
+// captured once into a HIP graph and replayed for any t.  The launch shape
+// of the step kernel is chosen by lbmc_batch_autotune (timed candidates).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -41,6 +47,10 @@ struct lbmc_batch {
     void *dmembers = nullptr;  // BatchMember<C>[count]
     int2 *chunks = nullptr;
     int nchunks = 0;
+    int2 *cchunks = nullptr;  // the closure members' chunks
+    int ncchunks = 0;
+    int eval_grid = 0;        // blocks of the evaluation kernel
+    int *any_eval = nullptr;  // device flag: some closure member evaluates this call
     const void *mlp = nullptr;  // closure weights shared by the closure members (a member's image)
     long long nodes = 0;
     int grid = 1;
@@ -48,6 +58,7 @@ struct lbmc_batch {
     int blocks_per_cu = 0;  // residency cap of the step kernel (0: none)
     int lds = 0;            //   as dynamic LDS bytes per block
     int lds_cu = 0;         // LDS bytes per CU
+    int ncu = 256;          // compute units of the device
     long long t_next = -1;  // the call the device arguments describe
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -93,14 +104,14 @@ template <class C>
 lbmc_status upload_members(lbmc_batch *b, long long t0, int store_every)
 {
     std::vector<BatchMember<C>> hm((size_t)b->count);
+    int any_eval = 0;  // the flag batch_advance_kernel keeps for later calls
     for (int k = 0; k < b->count; k++) {
         const lbmc_ctx *c = b->members[(size_t)k];
         if (c->nsf || c->key.wide || !c->fields_set)
             return bfail(b, LBMC_ERR_UNSUPPORTED, "batch member %d: sf nodes or state not supported", k);
         if (c->key.closure && (c->closure_model != b->closure_model || c->closure_lo != b->closure_lo ||
-                               c->closure_hi != b->closure_hi || c->closure_every != 1))
-            return bfail(b, LBMC_ERR_UNSUPPORTED,
-                         "batch member %d: closure members must share one model and band, interval 1", k);
+                               c->closure_hi != b->closure_hi))
+            return bfail(b, LBMC_ERR_UNSUPPORTED, "batch member %d: closure members must share one model and band", k);
         BatchMember<C> &m = hm[(size_t)k];
         memset(&m, 0, sizeof(m));
         const int se = store_every > 0 ? store_every : 0;
@@ -108,9 +119,17 @@ lbmc_status upload_members(lbmc_batch *b, long long t0, int store_every)
         m.variant = member_variant(b->members[(size_t)k]);
         m.store_every = se;
         m.store_count = se > 0 ? (int)((t0 + 1) % se) : 0;
+        // closure interval: the schedule of a single lattice (lbmc_run): the
+        // network runs on calls t % K == 0 and on the first call whose
+        // state has no valid record
+        m.closure_every = c->key.closure ? c->closure_every : 1;
+        m.closure_phase = (int)(t0 % m.closure_every);
+        if (m.closure_every > 1) m.a.tmode = (!c->tcache_valid || m.closure_phase == 0) ? 1 : 2;
+        if (m.variant >= 8 && m.a.tmode != 2) any_eval = 1;
     }
     B_HIP(b, hipMemcpyAsync(b->dmembers, hm.data(), sizeof(BatchMember<C>) * hm.size(), hipMemcpyHostToDevice,
                             b->stream));
+    B_HIP(b, hipMemcpyAsync(b->any_eval, &any_eval, sizeof(int), hipMemcpyHostToDevice, b->stream));
     B_HIP(b, hipStreamSynchronize(b->stream));
     b->t_next = t0;
     return LBMC_OK;
@@ -120,20 +139,20 @@ lbmc_status launch_one_step(lbmc_batch *b)
 {
     hipError_t e;
     const BatchShape sh{b->grid, b->sub, b->lds};
+    const BatchEval ev{b->cchunks, b->mlp ? b->ncchunks : 0, b->eval_grid, b->mlp, b->any_eval};
     switch (b->storage) {
     case LBMC_FP64: e = launch_batch_fp64(b->lattice, b->dmembers, b->chunks, b->nchunks, b->grid, b->sub, b->lds, b->stream); break;
-    case LBMC_FP32: e = launch_batch_st<StFP32>(b->lattice, b->dmembers, b->chunks, b->nchunks, sh, b->mlp, b->stream); break;
-    case LBMC_FP16S: e = launch_batch_st<StFP16S>(b->lattice, b->dmembers, b->chunks, b->nchunks, sh, b->mlp, b->stream); break;
-    default: e = launch_batch_st<StBF16S>(b->lattice, b->dmembers, b->chunks, b->nchunks, sh, b->mlp, b->stream); break;
+    case LBMC_FP32: e = launch_batch_st<StFP32>(b->lattice, b->dmembers, b->chunks, b->nchunks, sh, ev, b->stream); break;
+    case LBMC_FP16S: e = launch_batch_st<StFP16S>(b->lattice, b->dmembers, b->chunks, b->nchunks, sh, ev, b->stream); break;
+    default: e = launch_batch_st<StBF16S>(b->lattice, b->dmembers, b->chunks, b->nchunks, sh, ev, b->stream); break;
     }
     if (e != hipSuccess) return bfail(b, LBMC_ERR_DEVICE, "batch step launch: %s", hipGetErrorString(e));
-    const int nb = (b->count + 63) / 64;
     if (b->cdouble)
-        hipLaunchKernelGGL(batch_advance_kernel<double>, dim3(nb), dim3(64), 0, b->stream,
-                           (BatchMember<double> *)b->dmembers, b->count);
+        hipLaunchKernelGGL(batch_advance_kernel<double>, dim3(1), dim3(256), 0, b->stream,
+                           (BatchMember<double> *)b->dmembers, b->count, b->any_eval);
     else
-        hipLaunchKernelGGL(batch_advance_kernel<float>, dim3(nb), dim3(64), 0, b->stream,
-                           (BatchMember<float> *)b->dmembers, b->count);
+        hipLaunchKernelGGL(batch_advance_kernel<float>, dim3(1), dim3(256), 0, b->stream,
+                           (BatchMember<float> *)b->dmembers, b->count, b->any_eval);
     e = hipGetLastError();
     if (e != hipSuccess) return bfail(b, LBMC_ERR_DEVICE, "batch advance launch: %s", hipGetErrorString(e));
     return LBMC_OK;
@@ -188,22 +207,42 @@ lbmc_status batch_steps(lbmc_batch *b, long long t0, int nsteps)
     return LBMC_OK;
 }
 
-// Chunk list for `sub` tiles per chunk: (member, first node) of every chunk,
-// member by member.
+// Chunk lists for `sub` tiles per chunk: (member, first node) of every
+// chunk, member by member, and the same for the closure members alone.
 lbmc_status build_chunks(lbmc_batch *b, int sub)
 {
     const long long cn = (long long)sub * kBlock;
-    std::vector<int2> ch;
+    std::vector<int2> ch, cc;
     for (int k = 0; k < b->count; k++)
-        for (long long n = 0; n < b->members[(size_t)k]->N; n += cn) ch.push_back(make_int2(k, (int)n));
+        for (long long n = 0; n < b->members[(size_t)k]->N; n += cn) {
+            ch.push_back(make_int2(k, (int)n));
+            if (b->members[(size_t)k]->key.closure) cc.push_back(make_int2(k, (int)n));
+        }
     if (b->chunks) B_HIP(b, hipFree(b->chunks));
+    if (b->cchunks) B_HIP(b, hipFree(b->cchunks));
     b->chunks = nullptr;
+    b->cchunks = nullptr;
     B_HIP(b, hipMalloc((void **)&b->chunks, sizeof(int2) * ch.size()));
     B_HIP(b, hipMemcpy(b->chunks, ch.data(), sizeof(int2) * ch.size(), hipMemcpyHostToDevice));
+    if (!cc.empty()) {
+        B_HIP(b, hipMalloc((void **)&b->cchunks, sizeof(int2) * cc.size()));
+        B_HIP(b, hipMemcpy(b->cchunks, cc.data(), sizeof(int2) * cc.size(), hipMemcpyHostToDevice));
+    }
     b->nchunks = (int)ch.size();
+    b->ncchunks = (int)cc.size();
     b->sub = sub;
     b->grid = std::min(b->grid, b->nchunks);
+    // evaluation kernel: persistent, 4 blocks per CU (its residency)
+    b->eval_grid = std::max(1, std::min(b->ncchunks, 4 * b->ncu));
     return LBMC_OK;
+}
+
+// after calls were run: every closure member with an interval now holds the
+// record of its last evaluation
+void mark_records(lbmc_batch *b)
+{
+    for (lbmc_ctx *c : b->members)
+        if (c->key.closure && c->closure_every > 1) c->tcache_valid = true;
 }
 
 lbmc_status sync_members(lbmc_batch *b)
@@ -237,8 +276,6 @@ lbmc_status lbmc_batch_create(lbmc_ctx *const *members, int count, lbmc_batch **
                          "lbmc_batch_create: member %d differs in lattice, storage or device from member 0", k);
         if (c->nsf) return bfail(nullptr, LBMC_ERR_UNSUPPORTED, "lbmc_batch_create: member %d has sf boundary nodes", k);
         if (c->key.closure) {
-            if (c->desc.lattice != c0->desc.lattice || c->closure_every != 1)
-                return bfail(nullptr, LBMC_ERR_UNSUPPORTED, "lbmc_batch_create: member %d: closure with interval > 1", k);
             if (closure_model && (c->closure_model != closure_model || c->closure_lo != closure_lo ||
                                   c->closure_hi != closure_hi))
                 return bfail(nullptr, LBMC_ERR_UNSUPPORTED,
@@ -279,15 +316,15 @@ lbmc_status lbmc_batch_create(lbmc_ctx *const *members, int count, lbmc_batch **
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
     const size_t msz = b->cdouble ? sizeof(BatchMember<double>) : sizeof(BatchMember<float>);
     if (e == hipSuccess) e = hipMalloc(&b->dmembers, msz * (size_t)count);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->any_eval, sizeof(int));
     if (e != hipSuccess) return bail(bfail(b, LBMC_ERR_DEVICE, "lbmc_batch_create: %s", hipGetErrorString(e)));
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, b->device) == hipSuccess && prop.multiProcessorCount > 0)
+        b->ncu = prop.multiProcessorCount;
     b->grid = b->nchunks;
     if (lbmc_status st = build_chunks(b, 1)) return bail(st);
     // default persistent grid: 8 blocks per CU, at most one block per chunk
-    int ncu = 256;
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, b->device) == hipSuccess && prop.multiProcessorCount > 0)
-        ncu = prop.multiProcessorCount;
-    b->grid = (int)std::min<long long>(nchunks, 8LL * ncu);
+    b->grid = (int)std::min<long long>(b->nchunks, 8LL * b->ncu);
     if (hipDeviceGetAttribute(&b->lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, b->device) != hipSuccess)
         b->lds_cu = 0;
     *out = b;
@@ -302,6 +339,8 @@ lbmc_status lbmc_batch_destroy(lbmc_batch *b)
     drop_batch_graph(b);
     if (b->dmembers) (void)hipFree(b->dmembers);
     if (b->chunks) (void)hipFree(b->chunks);
+    if (b->cchunks) (void)hipFree(b->cchunks);
+    if (b->any_eval) (void)hipFree(b->any_eval);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -317,6 +356,7 @@ lbmc_status lbmc_batch_run(lbmc_batch *b, long long t0, int nsteps, int store_ev
     if (st == LBMC_OK) st = upload(b, t0, store_every);
     if (st == LBMC_OK) st = batch_steps(b, t0, nsteps);
     if (st != LBMC_OK) return st;
+    if (nsteps > 0) mark_records(b);
     B_HIP(b, hipStreamSynchronize(b->stream));
     return LBMC_OK;
 }
@@ -331,6 +371,7 @@ lbmc_status lbmc_batch_time_steps(lbmc_batch *b, long long t0, int nsteps, int s
     B_HIP(b, hipEventRecord(b->ev0, b->stream));
     st = batch_steps(b, t0, nsteps);
     if (st != LBMC_OK) return st;
+    mark_records(b);
     B_HIP(b, hipEventRecord(b->ev1, b->stream));
     B_HIP(b, hipEventSynchronize(b->ev1));
     B_HIP(b, hipEventElapsedTime(ms, b->ev0, b->ev1));

@@ -20,9 +20,12 @@ template <class C>
 struct BatchMember {
     StepArgs<C> a;
     int variant;
-    int store_every;  // 0: never store
-    int store_count;  // (t + 1) mod store_every for the current call t
-    int pad;
+    int store_every;    // 0: never store
+    int store_count;    // (t + 1) mod store_every for the current call t
+    int closure_every;  // closure members: the network runs on calls t with
+    int closure_phase;  //   t mod closure_every == 0 (phase = t mod every);
+                        //   a.tmode: 0 every call, 1 evaluate and record,
+                        //   2 reuse the record
 };
 
 template <class L, class St>
@@ -51,82 +54,138 @@ __device__ __forceinline__ void batch_node(const StepArgs<typename St::C> &a, in
 // chunks are read-only for the kernel's lifetime and reached through no
 // other pointer (__restrict__), so their loads are scalar and stay in scalar
 // registers across the population stores.
-// CLOSURE: some members use the learned closure (one shared network, staged
-// into LDS once per block); their tiles run the network over the whole
-// wave, lanes past the member's last node evaluating a copy of it.
-template <class L, class St, bool CLOSURE>
+//
+// A call is two kernels over disjoint sets of nodes.  batch_step_kernel
+// takes every chunk except those of closure members that evaluate the
+// network on this call: plain members, and closure members between
+// evaluations (recorded tau, no network).  batch_eval_kernel takes the
+// closure members' chunks and runs the network on those that evaluate; it
+// needs the weights in LDS and the matrix-core registers, which the first
+// kernel therefore does not carry.
+
+__device__ __forceinline__ bool evaluates(int variant, int tmode) { return variant >= 8 && tmode != 2; }
+
+template <class L, class St>
 __global__ __launch_bounds__(kBlock) void batch_step_kernel(const BatchMember<typename St::C> *__restrict__ members,
-                                                            const int2 *__restrict__ chunks, int nchunks,
-                                                            const void *__restrict__ mlp, int sub)
+                                                            const int2 *__restrict__ chunks, int nchunks, int sub)
 {
     using C = typename St::C;
-    constexpr bool CL = CLOSURE && std::is_same<C, float>::value;
-    __shared__ typename std::conditional<CL, MlpInfer, char>::type w;
-    __shared__ mlp_v8b xb[CL ? kBlock : 1];
-    if constexpr (CL) {
-        const uint4 *src = (const uint4 *)mlp;
-        uint4 *dst = (uint4 *)&w;
-        for (int k = (int)threadIdx.x; k < (int)(sizeof(MlpInfer) / 16); k += kBlock) dst[k] = src[k];
-        __syncthreads();
-    }
+    constexpr bool CL = std::is_same<C, float>::value;
     for (int c = (int)blockIdx.x; c < nchunks; c += (int)gridDim.x) {
         const int2 ch = chunks[c];  // (member, first node)
         const BatchMember<C> &m = members[ch.x];
         const StepArgs<C> &a = m.a;
         const int variant = m.variant;
+        if (evaluates(variant, a.tmode)) continue;  // batch_eval_kernel's
         for (int j = 0; j < sub; j++) {
             const int t0 = ch.y + j * kBlock;  // first node of the tile
             if (t0 >= a.nnodes) break;         // block-uniform
             const int n0 = t0 + (int)threadIdx.x;
+            if (n0 >= a.nnodes) continue;
+            int dd[3][3];
+            slot_offsets<L>(a, n0, dd);
             if constexpr (CL) {
                 if (variant >= 8) {
-                    const bool live = n0 < a.nnodes;
-                    const int n = live ? n0 : a.nnodes - 1;
-                    int dd[3][3];
-                    slot_offsets<L>(a, n, dd);
+                    // between evaluations: the recorded tau, no network
                     float v[L::Q];
-                    gather_node<L, St, false>(a, n, dd, v);
-                    if (variant == 9) closure_update<L, St, true, false>(a, n, live, v, w, xb + (threadIdx.x & ~63u));
-                    else closure_update<L, St, false, false>(a, n, live, v, w, xb + (threadIdx.x & ~63u));
-                    if (live) scatter_node<L, St, false>(a, n, dd, v);
+                    gather_node<L, St, false>(a, n0, dd, v);
+                    if (variant == 9) closure_reuse_node<L, St, true>(a, n0, v);
+                    else closure_reuse_node<L, St, false>(a, n0, v);
+                    scatter_node<L, St, false>(a, n0, dd, v);
                     continue;
                 }
             }
-            if (n0 < a.nnodes) {
-                int dd[3][3];
-                slot_offsets<L>(a, n0, dd);
-                batch_node<L, St>(a, variant, n0, dd);
-            }
+            batch_node<L, St>(a, variant, n0, dd);
         }
     }
 }
 
-// call t -> t + 1 for every member (one thread each)
-template <class C>
-__global__ void batch_advance_kernel(BatchMember<C> *members, int count)
+// Closure members' chunks (their own list) on a call where at least one of
+// them evaluates (*any_eval, written by batch_advance_kernel): the network
+// over whole waves, lanes past the member's last node evaluating a copy of
+// it and writing nothing.  The shared weights are staged into LDS once per
+// block.
+template <class L, class St>
+__global__ __launch_bounds__(kBlock, 4) void batch_eval_kernel(const BatchMember<float> *__restrict__ members,
+                                                               const int2 *__restrict__ chunks, int nchunks,
+                                                               const void *__restrict__ mlp, int sub,
+                                                               const int *__restrict__ any_eval)
 {
-    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (k >= count) return;
-    BatchMember<C> m = members[k];
-    long long stride = 1;
-    for (int d = 0; d < 3; d++) {
-        const int L = m.a.L[d];
-        int s = m.a.s[d] + 1;
-        if (s == L) s = 0;
-        m.a.s[d] = s;
-        m.a.dp[d] = (int)(-(long long)s * stride);
-        m.a.dpw[d] = (int)((long long)(L - s) * stride);
-        m.a.dm[d] = (int)((long long)s * stride);
-        m.a.dmw[d] = (int)((long long)(s - L) * stride);
-        stride *= L;
+    if (*any_eval == 0) return;
+    __shared__ MlpInfer w;
+    __shared__ mlp_v8b xb[kBlock];
+    {
+        const uint4 *src = (const uint4 *)mlp;
+        uint4 *dst = (uint4 *)&w;
+        for (int k = (int)threadIdx.x; k < (int)(sizeof(MlpInfer) / 16); k += kBlock) dst[k] = src[k];
     }
-    if (m.store_every > 0) {
-        int sc = m.store_count + 1;
-        if (sc == m.store_every) sc = 0;
-        m.store_count = sc;
-        m.a.store = sc == 0 ? 1 : 0;
+    __syncthreads();
+    mlp_v8b *xw = xb + (threadIdx.x & ~63u);
+    for (int c = (int)blockIdx.x; c < nchunks; c += (int)gridDim.x) {
+        const int2 ch = chunks[c];
+        const BatchMember<float> &m = members[ch.x];
+        const StepArgs<float> &a = m.a;
+        const int variant = m.variant;
+        if (!evaluates(variant, a.tmode)) continue;  // block-uniform
+        for (int j = 0; j < sub; j++) {
+            const int t0 = ch.y + j * kBlock;
+            if (t0 >= a.nnodes) break;
+            const int n0 = t0 + (int)threadIdx.x;
+            const bool live = n0 < a.nnodes;
+            const int n = live ? n0 : a.nnodes - 1;
+            int dd[3][3];
+            slot_offsets<L>(a, n, dd);
+            float v[L::Q];
+            gather_node<L, St, false>(a, n, dd, v);
+            if (a.tmode == 1) {
+                if (variant == 9) closure_update<L, St, true, true>(a, n, live, v, w, xw);
+                else closure_update<L, St, false, true>(a, n, live, v, w, xw);
+            } else {
+                if (variant == 9) closure_update<L, St, true, false>(a, n, live, v, w, xw);
+                else closure_update<L, St, false, false>(a, n, live, v, w, xw);
+            }
+            if (live) scatter_node<L, St, false>(a, n, dd, v);
+        }
     }
-    members[k] = m;
+}
+
+// call t -> t + 1 for every member (one block, a thread per member), and
+// *any_eval for call t + 1: whether some closure member evaluates then
+template <class C>
+__global__ __launch_bounds__(256) void batch_advance_kernel(BatchMember<C> *members, int count, int *any_eval)
+{
+    int ev = 0;
+    for (int k = (int)threadIdx.x; k < count; k += (int)blockDim.x) {
+        BatchMember<C> m = members[k];
+        long long stride = 1;
+        for (int d = 0; d < 3; d++) {
+            const int L = m.a.L[d];
+            int s = m.a.s[d] + 1;
+            if (s == L) s = 0;
+            m.a.s[d] = s;
+            m.a.dp[d] = (int)(-(long long)s * stride);
+            m.a.dpw[d] = (int)((long long)(L - s) * stride);
+            m.a.dm[d] = (int)((long long)s * stride);
+            m.a.dmw[d] = (int)((long long)(s - L) * stride);
+            stride *= L;
+        }
+        if (m.store_every > 0) {
+            int sc = m.store_count + 1;
+            if (sc == m.store_every) sc = 0;
+            m.store_count = sc;
+            m.a.store = sc == 0 ? 1 : 0;
+        }
+        if (m.closure_every > 1) {
+            int ph = m.closure_phase + 1;
+            if (ph == m.closure_every) ph = 0;
+            m.closure_phase = ph;
+            m.a.tmode = ph == 0 ? 1 : 2;
+        }
+        if (evaluates(m.variant, m.a.tmode)) ev = 1;
+        members[k] = m;
+    }
+    ev = __syncthreads_or(ev);
+    if (threadIdx.x == 0) *any_eval = ev;
 }
 
 // Launch shape of the batch step: grid blocks, `sub` 256-node tiles per
@@ -136,27 +195,37 @@ struct BatchShape {
     int grid, sub, lds;
 };
 
+// The closure members' evaluation: their chunk list (nchunks 0: no closure
+// members), its grid, the shared weights and the device flag
+struct BatchEval {
+    const int2 *chunks;
+    int nchunks, grid;
+    const void *mlp;
+    const int *any_eval;
+};
+
 template <class L, class St>
 hipError_t launch_batch_lattice(const void *members, const int2 *chunks, int nchunks, BatchShape sh,
-                                const void *mlp, hipStream_t s)
+                                const BatchEval &ev, hipStream_t s)
 {
-    const auto *mm = (const BatchMember<typename St::C> *)members;
-    if (mlp)
-        hipLaunchKernelGGL((batch_step_kernel<L, St, true>), dim3(sh.grid), dim3(kBlock), sh.lds, s, mm, chunks,
-                           nchunks, mlp, sh.sub);
-    else
-        hipLaunchKernelGGL((batch_step_kernel<L, St, false>), dim3(sh.grid), dim3(kBlock), sh.lds, s, mm, chunks,
-                           nchunks, mlp, sh.sub);
+    using C = typename St::C;
+    hipLaunchKernelGGL((batch_step_kernel<L, St>), dim3(sh.grid), dim3(kBlock), sh.lds, s,
+                       (const BatchMember<C> *)members, chunks, nchunks, sh.sub);
+    if constexpr (std::is_same<C, float>::value) {
+        if (ev.nchunks > 0)
+            hipLaunchKernelGGL((batch_eval_kernel<L, St>), dim3(ev.grid), dim3(kBlock), 0, s,
+                               (const BatchMember<float> *)members, ev.chunks, ev.nchunks, ev.mlp, sh.sub,
+                               ev.any_eval);
+    }
     return hipGetLastError();
 }
 
-// mlp: the shared closure weights when some member uses the closure, else null
 template <class St>
 hipError_t launch_batch_st(int lattice, const void *members, const int2 *chunks, int nchunks, BatchShape sh,
-                           const void *mlp, hipStream_t s)
+                           const BatchEval &ev, hipStream_t s)
 {
-    return lattice == 0 ? launch_batch_lattice<D2Q9, St>(members, chunks, nchunks, sh, mlp, s)
-                        : launch_batch_lattice<D3Q15, St>(members, chunks, nchunks, sh, mlp, s);
+    return lattice == 0 ? launch_batch_lattice<D2Q9, St>(members, chunks, nchunks, sh, ev, s)
+                        : launch_batch_lattice<D3Q15, St>(members, chunks, nchunks, sh, ev, s);
 }
 
 }  // namespace lbmc

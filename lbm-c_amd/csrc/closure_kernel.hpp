@@ -8,8 +8,9 @@
 //  * every node of a wave evaluates the network together, in uniform control
 //    flow (the matrix-core operands span the wave); lanes past the last node
 //    evaluate a copy of the last node and write nothing;
-//  * plain fluid nodes relax with tau' = out_a * y + out_b, or with the
-//    Smagorinsky tau where tau' leaves [lo, hi] (tau when c_smag = 0);
+//  * plain fluid nodes relax with tau' = out_a * y + out_b + shift (the
+//    lattice's shift, 0 by default), or with the Smagorinsky tau where tau'
+//    leaves [lo, hi] (tau when c_smag = 0);
 //    boundary and porous nodes take the reference-order update with the
 //    Smagorinsky hook.
 #pragma once
@@ -24,12 +25,12 @@ __device__ __forceinline__ void closure_update(const StepArgs<float> &a, int n, 
                                                const MlpInfer &w, mlp_v8b *xw)
 {
     constexpr int D = L::D;
-    const unsigned m = a.mask[n];
+    const unsigned m = ldg(a.mask + n);
     float F[3] = {0.f, 0.f, 0.f};
     if constexpr (GUO) {
         sfor<0, D>([&](auto dd) {
             constexpr int d = decltype(dd)::value;
-            F[d] = a.force[0] ? a.force[d][n] : a.F[d];
+            F[d] = a.force[0] ? ldg(a.force[d] + n) : a.F[d];
         });
     }
     FastState<L> st;
@@ -39,17 +40,49 @@ __device__ __forceinline__ void closure_update(const StepArgs<float> &a, int n, 
     const float y = mlp_wave_infer(w, xw, mlp_input(w.sc, x));
     if (!live) return;
     if (m == 0) {
-        float t = w.sc.out_a * y + w.sc.out_b;
+        float t = w.sc.out_a * y + w.sc.out_b + a.closure_shift;
         const bool in_band = t >= w.sc.lo && t <= w.sc.hi;
-        if constexpr (RECORD) a.tcache[n] = in_band ? t : __builtin_nanf("");
+        if constexpr (RECORD) stg(a.tcache + n, in_band ? t : __builtin_nanf(""));
         if (!in_band) t = a.c_smag > 0.f ? fast_smag_tau<L, St::shifted>(st, a.tau, a.c_smag) : a.tau;
         fast_relax<L, St::shifted, GUO>(v, st, t, a.tau, F);
         if (a.store) {
             sfor<0, D>([&](auto dd) {
                 constexpr int d = decltype(dd)::value;
-                a.u[d][n] = st.u[d];
+                stg(a.u[d] + n, st.u[d]);
             });
-            a.rho[n] = st.rho;
+            stg(a.rho + n, st.rho);
+        }
+    } else {
+        node_update<L, St, 0, GUO, true, false>(a, n, v);
+    }
+}
+
+// A call between evaluations, one node: plain fluid nodes relax with the
+// recorded tau (the Smagorinsky tau of the current state where the record is
+// NaN); no network.
+template <class L, class St, bool GUO>
+__device__ __forceinline__ void closure_reuse_node(const StepArgs<float> &a, int n, float (&v)[L::Q])
+{
+    if (ldg(a.mask + n) == 0) {
+        constexpr int D = L::D;
+        float F[3] = {0.f, 0.f, 0.f};
+        if constexpr (GUO) {
+            sfor<0, D>([&](auto dd2) {
+                constexpr int d = decltype(dd2)::value;
+                F[d] = a.force[0] ? ldg(a.force[d] + n) : a.F[d];
+            });
+        }
+        FastState<L> st;
+        fast_prepare<L, St::shifted, GUO>(v, F, st);
+        float t = ldg(a.tcache + n);
+        if (t != t) t = a.c_smag > 0.f ? fast_smag_tau<L, St::shifted>(st, a.tau, a.c_smag) : a.tau;
+        fast_relax<L, St::shifted, GUO>(v, st, t, a.tau, F);
+        if (a.store) {
+            sfor<0, D>([&](auto dd2) {
+                constexpr int d = decltype(dd2)::value;
+                stg(a.u[d] + n, st.u[d]);
+            });
+            stg(a.rho + n, st.rho);
         }
     } else {
         node_update<L, St, 0, GUO, true, false>(a, n, v);
@@ -82,9 +115,8 @@ __global__ __launch_bounds__(kBlock, 4) void closure_step_kernel(const StepArgs<
     }
 }
 
-// A call between evaluations: plain fluid nodes relax with the recorded tau
-// (the Smagorinsky tau of the current state where the record is NaN); one
-// thread per node as in step_kernel, no network.  4 more bytes per node.
+// A call between evaluations: closure_reuse_node at every node, one thread
+// per node as in step_kernel; 4 more bytes per node (the record).
 template <class L, class St, bool GUO, bool WIDE>
 __global__ __launch_bounds__(kBlock) void closure_reuse_kernel(const StepArgs<float> a)
 {
@@ -94,30 +126,7 @@ __global__ __launch_bounds__(kBlock) void closure_reuse_kernel(const StepArgs<fl
     slot_offsets<L>(a, n, dd);
     float v[L::Q];
     gather_node<L, St, WIDE>(a, n, dd, v);
-    if (a.mask[n] == 0) {
-        constexpr int D = L::D;
-        float F[3] = {0.f, 0.f, 0.f};
-        if constexpr (GUO) {
-            sfor<0, D>([&](auto dd2) {
-                constexpr int d = decltype(dd2)::value;
-                F[d] = a.force[0] ? a.force[d][n] : a.F[d];
-            });
-        }
-        FastState<L> st;
-        fast_prepare<L, St::shifted, GUO>(v, F, st);
-        float t = a.tcache[n];
-        if (t != t) t = a.c_smag > 0.f ? fast_smag_tau<L, St::shifted>(st, a.tau, a.c_smag) : a.tau;
-        fast_relax<L, St::shifted, GUO>(v, st, t, a.tau, F);
-        if (a.store) {
-            sfor<0, D>([&](auto dd2) {
-                constexpr int d = decltype(dd2)::value;
-                a.u[d][n] = st.u[d];
-            });
-            a.rho[n] = st.rho;
-        }
-    } else {
-        node_update<L, St, 0, GUO, true, false>(a, n, v);
-    }
+    closure_reuse_node<L, St, GUO>(a, n, v);
     scatter_node<L, St, WIDE>(a, n, dd, v);
 }
 

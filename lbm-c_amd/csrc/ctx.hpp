@@ -51,6 +51,7 @@ struct lbmc_ctx {
     const lbmc_mlp *closure_model = nullptr;  // the model the image was built from
     double closure_lo = 0, closure_hi = 0;
     int closure_every = 1;
+    double closure_shift = 0;  // added to the network's tau' on this lattice
     float *tcache = nullptr;
     bool tcache_valid = false;
 
@@ -120,6 +121,7 @@ inline void fill_args(const lbmc_ctx *c, StepArgs<C> &a, long long t, int store)
     a.mlp = c->mlp_img;
     a.tcache = c->tcache;
     a.tmode = 0;
+    a.closure_shift = (C)c->closure_shift;
 }
 
 }  // namespace lbmc

@@ -764,6 +764,16 @@ lbmc_status lbmc_set_closure_interval(lbmc_ctx *c, int every)
     return LBMC_OK;
 }
 
+lbmc_status lbmc_set_closure_shift(lbmc_ctx *c, double dtau)
+{
+    if (!c) return fail(nullptr, LBMC_ERR_INVALID, "lbmc_set_closure_shift: null context");
+    if (!std::isfinite(dtau)) return fail(c, LBMC_ERR_INVALID, "lbmc_set_closure_shift: shift not finite");
+    c->closure_shift = dtau;
+    invalidate_tau_record(c);
+    drop_graph(c);
+    return LBMC_OK;
+}
+
 lbmc_status lbmc_sync(lbmc_ctx *c)
 {
     if (!c) return fail(nullptr, LBMC_ERR_INVALID, "null context");

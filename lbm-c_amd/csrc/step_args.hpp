@@ -74,6 +74,7 @@ struct StepArgs {
     const void *mlp;         // closure weights (MlpImage) when the closure is on
     float *tcache;           // closure: per-node tau of the last evaluation (NaN: fall back)
     int tmode;               // closure: 0 evaluate, 1 evaluate and record, 2 use the record
+    C closure_shift;         // closure: added to the network's tau'
 };
 
 }  // namespace lbmc

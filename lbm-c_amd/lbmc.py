@@ -39,6 +39,7 @@ EXPORTS = [
     "lbmc_mlp_create", "lbmc_mlp_destroy", "lbmc_mlp_last_error", "lbmc_mlp_set_params",
     "lbmc_mlp_get_params", "lbmc_mlp_set_scaling", "lbmc_mlp_forward", "lbmc_mlp_loss_grad",
     "lbmc_mlp_set_data", "lbmc_mlp_train", "lbmc_set_closure", "lbmc_set_closure_interval",
+    "lbmc_set_closure_shift",
     "lbmc_batch_create", "lbmc_batch_destroy", "lbmc_batch_last_error", "lbmc_batch_run",
     "lbmc_batch_time_steps", "lbmc_batch_autotune", "lbmc_batch_set_shape", "lbmc_batch_info",
 ]
@@ -121,6 +122,7 @@ def _load():
                                       ct.POINTER(ct.c_double)]),
         "lbmc_set_closure": (ct.c_int, [P, P, ct.c_double, ct.c_double]),
         "lbmc_set_closure_interval": (ct.c_int, [P, ct.c_int]),
+        "lbmc_set_closure_shift": (ct.c_int, [P, ct.c_double]),
         "lbmc_batch_create": (ct.c_int, [ct.POINTER(P), ct.c_int, ct.POINTER(P)]),
         "lbmc_batch_destroy": (ct.c_int, [P]),
         "lbmc_batch_last_error": (ct.c_char_p, [P]),
@@ -331,6 +333,10 @@ class Lattice:
         """Evaluate the closure network every `every` calls (reusing each
         node's recorded tau in between)."""
         self._check(_lib.lbmc_set_closure_interval(self._h, int(every)))
+
+    def set_closure_shift(self, dtau):
+        """Add dtau to the network's tau' on this lattice (before the band check)."""
+        self._check(_lib.lbmc_set_closure_shift(self._h, float(dtau)))
 
 
 def copy_bandwidth(nbytes, reps=20, device=0):

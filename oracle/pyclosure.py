@@ -9,7 +9,7 @@ optional uniform Guo force, the reference's call semantics
 pulls f_i(x) <- f_i(x - e_i) and then collides; solids bounce back
 (LBM-C/collision.cu:423-443).  At fluid nodes tau comes from the network
 (oracle/pymlp.py, bf16 quantisation points emulated) as out_a * y + out_b
-(evaluated on calls t % every == 0 and the first call, the recorded tau
++ shift (shift: the lattice's offset, 0 by default; evaluated on calls t % every == 0 and the first call, the recorded tau
 reused in between), or from the Smagorinsky formula (LBM-C/collision.cu:445-463, restated in
 closed form: tau' = (tau + sqrt(tau^2 + 36 C^2 (sum|Pi_aa| + 2 sum_{a<b}
 |Pi_ab|) / rho)) / 2) when that leaves [lo, hi].  The Guo terms follow
@@ -27,7 +27,7 @@ import pyoracle
 
 class ClosureOracle:
     def __init__(self, lattice, length, tau, c_smag, params, mu, isd, out_a, out_b, lo, hi,
-                 force=None, solid=None, quant=True, every=1):
+                 force=None, solid=None, quant=True, every=1, shift=0.0):
         self.lattice = lattice
         self.dim = 2 if lattice == pyoracle.D2Q9 else 3
         e, w, opp, _, _ = pyoracle.tables(lattice)
@@ -41,6 +41,7 @@ class ClosureOracle:
         self.mu = np.asarray(mu, dtype=np.float32)
         self.isd = np.asarray(isd, dtype=np.float32)
         self.out_a, self.out_b = float(out_a), float(out_b)
+        self.shift = float(shift)
         self.lo, self.hi = float(lo), float(hi)
         self.F = np.zeros(self.dim) if force is None else np.asarray(force, dtype=np.float64)[: self.dim]
         self.guo = force is not None
@@ -103,7 +104,7 @@ class ClosureOracle:
         if self.every <= 1 or self.record is None or self.calls % self.every == 0:
             xn = ((x.astype(np.float32) - self.mu) * self.isd).astype(np.float64)
             y, _ = pymlp.forward(self.p, xn, quant=self.quant)
-            t = self.out_a * y + self.out_b
+            t = self.out_a * y + self.out_b + self.shift
             self.tau_net = t
             bad = ~((t >= self.lo) & (t <= self.hi))
             self.record = np.where(bad, np.nan, t)

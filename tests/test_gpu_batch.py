@@ -133,10 +133,13 @@ def test_batch_rejects_mixed_members():
         x.close()
 
 
-def test_batch_with_closure_members_matches_single_lattices():
+@pytest.mark.parametrize("every", [1, 3])
+def test_batch_with_closure_members_matches_single_lattices(every):
     """D2Q9 members with the learned closure (one shared network; with and
-    without Guo forcing) beside an MRT member: each ends where it ends when
-    stepped on its own."""
+    without Guo forcing; the network evaluated every call or every 3rd, with
+    the recorded tau in between) beside an MRT member: each ends where it
+    ends when stepped on its own, over two runs (the second starting from a
+    valid record at a call that is not a multiple of 3)."""
     import pymlp
     import workloads
 
@@ -167,20 +170,25 @@ def test_batch_with_closure_members_matches_single_lattices():
         lat.init_equilibrium(np.ones(n), spec["u"])
         if spec["closure"]:
             lat.set_closure(model, 0.6, 0.7)
+            lat.set_closure_interval(every)
         return lat
 
-    steps = 50
+    runs = [(0, 50), (50, 23)]
     lats = [build(s) for s in specs]
     b = lbmc.Batch(lats)
-    b.run(0, steps, 1)
-    got = [lat.get_f() for lat in lats]
+    for t0, n in runs:
+        b.run(t0, n, 1)
+    got = [(lat.get_f(), lat.get_macros()[1]) for lat in lats]
     b.close()
     for lat in lats:
         lat.close()
-    for spec, fg in zip(specs, got):
+    for spec, (fg, ug) in zip(specs, got):
         lat = build(spec)
-        lat.run(0, steps, 1)
+        for t0, n in runs:
+            lat.run(t0, n, 1)
         fs = lat.get_f()
+        us = lat.get_macros()[1]
         lat.close()
         assert np.max(np.abs(fg - fs)) <= 1e-6, spec["L"]
+        assert np.max(np.abs(ug - us)) <= 1e-6, spec["L"]
     model.close()

@@ -131,7 +131,7 @@ MU = np.array([0, 0, 0, 0, 0, 0, 1.0], dtype=np.float32)
 ISD = np.array([300.0] * 6 + [100.0], dtype=np.float32)
 
 
-def _pair(lattice, L, tau, c_smag, storage, band, out, force=None, solid=None, seed=20, every=1):
+def _pair(lattice, L, tau, c_smag, storage, band, out, force=None, solid=None, seed=20, every=1, shift=0.0):
     """The GPU lattice and the oracle, same initial state and network."""
     dim = 2 if lattice == lbmc.D2Q9 else 3
     n = int(np.prod(L))
@@ -150,8 +150,10 @@ def _pair(lattice, L, tau, c_smag, storage, band, out, force=None, solid=None, s
     lat.set_closure(m, band[0], band[1])
     if every > 1:
         lat.set_closure_interval(every)
+    if shift:
+        lat.set_closure_shift(shift)
     co = pyclosure.ClosureOracle(lattice, L, tau, c_smag, p, MU, ISD, out[0], out[1], band[0], band[1],
-                                 force=force, solid=solid, every=every)
+                                 force=force, solid=solid, every=every, shift=shift)
     co.init_equilibrium(rho, u)
     return lat, m, co
 
@@ -199,12 +201,13 @@ def test_closure_single_call_matches_oracle(lattice):
     m.close()
 
 
-def _run_pair(lattice, L, tau, c_smag, storage, steps, force=None, solid=None, every=1):
+def _run_pair(lattice, L, tau, c_smag, storage, steps, force=None, solid=None, every=1, shift=0.0):
     """Network output well inside the band everywhere (no branch near its
-    edges): the macros after `steps` calls."""
+    edges): the macros after `steps` calls.  With a shift, the model's own
+    output is centred on tau + 0.03 - shift."""
     band = (tau + 0.001, tau + 0.06)
-    lat, m, co = _pair(lattice, L, tau, c_smag, storage, band, (0.004, tau + 0.03), force=force, solid=solid,
-                       every=every)
+    lat, m, co = _pair(lattice, L, tau, c_smag, storage, band, (0.004, tau + 0.03 - shift), force=force,
+                       solid=solid, every=every, shift=shift)
     lat.run(0, steps, 1)
     rd, ud = lat.get_macros()
     ro, uo = co.run(steps)
@@ -230,6 +233,20 @@ def test_closure_steps_d2q9_guo_solids_match_oracle():
     g[10:14, 18:22] = 1
     (rd, ud), (ro, uo), fb, edge = _run_pair(lbmc.D2Q9, (40, 24), 0.6, 0.1, lbmc.FP32, 15, force=[2e-5, 0.0],
                                              solid=g.reshape(-1))
+    assert edge > 1e-3
+    assert np.max(np.abs(ud - uo)) < 1e-5
+    assert np.max(np.abs(rd - ro)) < 1e-5
+
+
+def test_closure_shift_matches_oracle():
+    """A model centred on tau' = 0.33 serving a tau = 0.6 lattice through a
+    shift of 0.3 (Guo forcing, solids)."""
+    print("[closure] shift", flush=True)
+    g = np.zeros((24, 40))
+    g[0, :] = g[-1, :] = 1
+    g[10:14, 18:22] = 1
+    (rd, ud), (ro, uo), fb, edge = _run_pair(lbmc.D2Q9, (40, 24), 0.6, 0.1, lbmc.FP32, 15, force=[2e-5, 0.0],
+                                             solid=g.reshape(-1), shift=0.3)
     assert edge > 1e-3
     assert np.max(np.abs(ud - uo)) < 1e-5
     assert np.max(np.abs(rd - ro)) < 1e-5
