@@ -137,9 +137,9 @@ def main():
     if not args.no_variants:
         for st in (lbmc.FP16S, lbmc.BF16S):
             v = make_lattice(lbmc, L, st, fields)
-            v.time_steps(0, max(5, args.warmup // 5), 0)
-            vms = v.time_steps(args.warmup, max(20, args.steps // 5), 0)
-            vsteps = max(20, args.steps // 5)
+            vsteps = max(20, args.steps)
+            v.time_steps(0, max(5, args.warmup), 0)
+            vms = v.time_steps(args.warmup, vsteps, 0)
             vb = 2 * 15 * 2 + 1
             variants[lbmc.STORAGE_NAMES[st]] = {
                 "MLUPS": round(N * vsteps / (vms * 1e-3) / 1e6, 1),

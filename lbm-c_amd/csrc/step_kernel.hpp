@@ -317,21 +317,31 @@ __device__ __forceinline__ void gather_node(const StepArgs<typename St::C> &a, i
     });
 }
 
-// write-back to the same slots (solver.cu:119-123)
+// write-back to the same slots (solver.cu:119-123); the 16-bit formats
+// round stochastically (storage.hpp)
 template <class L, class St, bool WIDE>
 __device__ __forceinline__ void scatter_node(const StepArgs<typename St::C> &a, int n, const int (&dd)[3][3],
                                              const typename St::C (&v)[L::Q])
 {
     using S = typename St::S;
     S *const fb = (S *)a.f;
+    unsigned r = 0;
+    if constexpr (St::stochastic) r = sr_seed(n, a.tseed);
     sfor<0, L::Q>([&](auto ii) {
         constexpr int i = decltype(ii)::value;
         const int k = n + dd[0][L::ex[i] + 1] + dd[1][L::ey[i] + 1] + dd[2][L::ez[i] + 1];
+        S s;
+        if constexpr (St::stochastic) {
+            s = St::store_sr(v[i], r);
+            r = sr_next(r);
+        } else {
+            s = St::store(v[i]);
+        }
         if constexpr (WIDE) {
-            stg(fb + (size_t)i * a.npad + k, St::store(v[i]));
+            stg(fb + (size_t)i * a.npad + k, s);
         } else {
             char *base = (char *)(fb + (size_t)i * a.npad);
-            stg((S *)(base + (unsigned)k * (unsigned)sizeof(S)), St::store(v[i]));
+            stg((S *)(base + (unsigned)k * (unsigned)sizeof(S)), s);
         }
     });
 }

@@ -2,14 +2,23 @@
 //
 //   FP64  : double slots, double arithmetic (the reference's precision)
 //   FP32  : float slots holding g_i = f_i - w_i, float arithmetic
-//   FP16S : _Float16 slots holding g_i = f_i - w_i, float arithmetic
-//   BF16S : __bf16 slots holding g_i = f_i - w_i, float arithmetic
+//   FP16S : _Float16 slots holding 2^12 g_i, float arithmetic
+//   BF16S : __bf16 slots holding g_i, float arithmetic
 //
 // The shifted formats store the deviation from the rest equilibrium so that
 // the 24-, 11- or 8-bit significand is spent on the part of f that varies
 // (|g| ~ 1e-3 w against |f| ~ w at low Mach number: for FP32 the rounding of a
 // slot drops from ~3e-8 to ~1e-10); density is accumulated as 1 + sum g_i and
 // momentum as sum e_i g_i (the rest weights sum to one and carry no momentum).
+// FP16S scales g by 2^12 so that deviations down to ~1e-8 stay in the
+// normal range of _Float16 (|g| up to 16 fits).
+//
+// The 16-bit formats round the step's stores stochastically (store_sr, the
+// hardware's stochastic-rounding conversion, which draws on the high bits of
+// a random word): a slot then holds its value's expectation, so increments
+// below half a unit in the last place -- a weak body force adds ~1e-7 per
+// call to populations of ~1e-3 -- accumulate instead of being rounded away
+// call after call.  Set-up writes (initial states, set_f) round to nearest.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,6 +31,7 @@ struct StFP64 {
     using S = double;
     using C = double;
     static constexpr bool shifted = false;
+    static constexpr bool stochastic = false;
     static constexpr int id = ST_FP64;
     __device__ static inline C load(S v) { return v; }
     __device__ static inline S store(C v) { return v; }
@@ -31,6 +41,7 @@ struct StFP32 {
     using S = float;
     using C = float;
     static constexpr bool shifted = true;
+    static constexpr bool stochastic = false;
     static constexpr int id = ST_FP32;
     __device__ static inline C load(S v) { return v; }
     __device__ static inline S store(C v) { return v; }
@@ -40,18 +51,47 @@ struct StFP16S {
     using S = _Float16;
     using C = float;
     static constexpr bool shifted = true;
+    static constexpr bool stochastic = true;
     static constexpr int id = ST_FP16S;
-    __device__ static inline C load(S v) { return (float)v; }
-    __device__ static inline S store(C v) { return (_Float16)v; }  // round to nearest even
+    static constexpr float kScale = 4096.f, kInv = 1.f / 4096.f;
+    __device__ static inline C load(S v) { return (float)v * kInv; }
+    __device__ static inline S store(C v) { return (_Float16)(v * kScale); }  // round to nearest even
+    __device__ static inline S store_sr(C v, unsigned r)
+    {
+        typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+        return __builtin_amdgcn_cvt_sr_f16_f32((h2){0, 0}, v * kScale, r, false)[0];
+    }
 };
 
 struct StBF16S {
     using S = __bf16;
     using C = float;
     static constexpr bool shifted = true;
+    static constexpr bool stochastic = true;
     static constexpr int id = ST_BF16S;
     __device__ static inline C load(S v) { return (float)v; }
     __device__ static inline S store(C v) { return (__bf16)v; }  // round to nearest even
+    __device__ static inline S store_sr(C v, unsigned r)
+    {
+        typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+        return __builtin_amdgcn_cvt_sr_bf16_f32((b2){0, 0}, v, r, false)[0];
+    }
 };
+
+// Random words of the stochastic stores: a hash of (node, call seed), then
+// one step of a 32-bit linear congruential generator per population (the
+// conversions use the high bits, the LCG's good ones).
+__device__ __forceinline__ unsigned sr_seed(int n, unsigned tseed)
+{
+    unsigned h = (unsigned)n * 0x9E3779B1u ^ tseed * 0x85EBCA77u;
+    h ^= h >> 16;
+    h *= 0x7FEB352Du;
+    h ^= h >> 15;
+    h *= 0x846CA68Bu;
+    h ^= h >> 16;
+    return h;
+}
+
+__device__ __forceinline__ unsigned sr_next(unsigned h) { return h * 1664525u + 1013904223u; }
 
 }  // namespace lbmc

@@ -1,14 +1,23 @@
 #!/usr/bin/env python3
 """Batched ensemble of uneven D2Q9 lattices (workloads.ensemble_d2q9):
-aggregate throughput of the persistent batched step, and each member's
-error against the same ensemble stepped with FP64 storage (whose kernels
-match the FP64 CPU restatement of the reference bit for bit).
+aggregate throughput of the batched step and each member's error against a
+reference ensemble.
 
-  1. build the members (FP32, FP16S, BF16S and FP64 storage), static start;
-  2. autotune the persistent grid on a throw-away copy, then time K steps;
-  3. run every storage N steps (default 10,000) and compare u per member:
-     relative L2 error ||u - u_FP64|| / ||u_FP64|| over fluid nodes, and the
-     member's Reynolds number from its mean velocity.
+Members are MRT or BGK; in the "learned" mix the BGK members use the learned
+relaxation-time closure (one shared, He-initialised network whose output
+tau' = 0.55 + 0.01 y is shifted to each member's tau; fallback band
+[0.5005, 2.0]), evaluated every call or every 8th call.
+
+  1. build the members, static start; autotune the launch shape on a
+     throw-away copy, then time K steps: the plain mix at FP32, FP16S,
+     BF16S and FP64 storage, the learned mix at FP32, FP16S and BF16S;
+  2. run the plain mix N steps (default 10,000) at every storage and compare
+     u per member against FP64 storage (whose kernels match the FP64 CPU
+     restatement of the reference bit for bit): relative L2 error over fluid
+     nodes, and the member's Reynolds number from its mean velocity;
+  3. run the learned mix N steps at FP32, FP16S and BF16S and compare the
+     16-bit formats against FP32 (the closure needs FP32 arithmetic, so
+     there is no FP64 run of it).
 
 Writes one JSON document (--out).
 """
@@ -31,7 +40,19 @@ def log(msg):
     print(f"[ensemble {time.strftime('%H:%M:%S')}] {msg}", flush=True)
 
 
-def build(specs, storage):
+TAU_REF = 0.55  # the shared network's own tau' centre
+
+
+def make_model():
+    import closure
+
+    mu = np.array([0, 0, 0, 0, 0, 0, 1.0], dtype=np.float32)
+    isd = np.array([1e4] * 6 + [100.0], dtype=np.float32)
+    return lbmc.Mlp(closure.init_params(0), mu=mu, isd=isd, out_a=0.01, out_b=TAU_REF)
+
+
+def build(specs, storage, model=None, every=1):
+    """The members; with a model, BGK members use the learned closure."""
     lats = []
     for m in specs:
         Lx, Ly = m["length"]
@@ -41,8 +62,21 @@ def build(specs, storage):
         n = Lx * Ly
         lat.set_fields(geometry=m["geometry"].reshape(-1), force=[np.full(n, m["force"]), np.zeros(n)])
         lat.init_static()
+        if model is not None and coll == lbmc.BGK:
+            lat.set_closure(model, 0.5005, 2.0)
+            lat.set_closure_shift(m["tau"] - TAU_REF)
+            lat.set_closure_interval(every)
         lats.append(lat)
     return lats
+
+
+def member_errors(specs, got, ref):
+    e = []
+    for k, m in enumerate(specs):
+        fluid = m["geometry"].reshape(-1) < 1
+        d = got[k][:, fluid] - ref[k][:, fluid]
+        e.append(float(np.linalg.norm(d) / max(np.linalg.norm(ref[k][:, fluid]), 1e-300)))
+    return {"max": max(e), "median": float(np.median(e)), "per_member": e}
 
 
 def main():
@@ -61,13 +95,13 @@ def main():
            "tau": [m["tau"] for m in specs], "collision": [m["collision"] for m in specs]}
     log(f"{args.count} members, {N} nodes")
 
-    # ---- throughput (autotuned grid), per storage
-    thr = {}
-    for st in (lbmc.FP32, lbmc.FP16S, lbmc.BF16S, lbmc.FP64):
-        name = lbmc.STORAGE_NAMES[st]
-        lats = build(specs, st)
+    model = make_model()
+
+    # ---- throughput (autotuned launch shape)
+    def timed(st, learned_every=0):
+        lats = build(specs, st, model if learned_every else None, learned_every or 1)
         b = lbmc.Batch(lats)
-        grid, tune_ms = b.autotune(0, steps_per_trial=20)
+        grid, _ = b.autotune(0, steps_per_trial=20)
         shape = b.shape()
         b.close()
         for lat in lats:
@@ -80,11 +114,23 @@ def main():
         b.close()
         for lat in lats:
             lat.close()
-        thr[name] = {"MLUPS": round(N * args.time_steps / (ms * 1e-3) / 1e6, 1),
-                     "ms_per_step": round(ms / args.time_steps, 5), "grid": grid, "chunks": info["chunks"],
-                     "chunk_nodes": info["chunk_nodes"], "blocks_per_cu": info["blocks_per_cu"]}
-        log(f"{name}: {thr[name]}")
+        return {"MLUPS": round(N * args.time_steps / (ms * 1e-3) / 1e6, 1),
+                "ms_per_step": round(ms / args.time_steps, 5), "grid": grid, "chunks": info["chunks"],
+                "chunk_nodes": info["chunk_nodes"], "blocks_per_cu": info["blocks_per_cu"]}
+
+    thr = {}
+    for st in (lbmc.FP32, lbmc.FP16S, lbmc.BF16S, lbmc.FP64):
+        name = lbmc.STORAGE_NAMES[st]
+        thr[name] = timed(st)
+        log(f"plain {name}: {thr[name]}")
     res["throughput"] = thr
+    thr_l = {}
+    for st in (lbmc.FP32, lbmc.FP16S, lbmc.BF16S):
+        name = lbmc.STORAGE_NAMES[st]
+        for every in (1, 8):
+            thr_l[f"{name}_every_{every}"] = timed(st, every)
+            log(f"learned {name} every {every}: {thr_l[f'{name}_every_{every}']}")
+    res["throughput_learned"] = thr_l
 
     # ---- accuracy after the full run, against FP64
     finals = {}
@@ -103,14 +149,27 @@ def main():
     errs = {}
     ref = finals["f64"]
     for name in ("f32", "f16s", "bf16s"):
-        e = []
-        for k, m in enumerate(specs):
-            fluid = m["geometry"].reshape(-1) < 1
-            d = finals[name][k][:, fluid] - ref[k][:, fluid]
-            e.append(float(np.linalg.norm(d) / max(np.linalg.norm(ref[k][:, fluid]), 1e-300)))
-        errs[name] = {"max": max(e), "median": float(np.median(e)), "per_member": e}
-        log(f"{name} vs f64: max rel L2 {max(e):.3e}, median {np.median(e):.3e}")
+        errs[name] = member_errors(specs, finals[name], ref)
+        log(f"{name} vs f64: max rel L2 {errs[name]['max']:.3e}, median {errs[name]['median']:.3e}")
     res["error_vs_f64"] = errs
+
+    # ---- the learned mix (network every call): 16-bit formats against FP32
+    finals_l = {}
+    for st in (lbmc.FP32, lbmc.FP16S, lbmc.BF16S):
+        name = lbmc.STORAGE_NAMES[st]
+        lats = build(specs, st, model, 1)
+        b = lbmc.Batch(lats)
+        b.run(0, args.steps, args.steps)
+        finals_l[name] = [lat.get_macros()[1] for lat in lats]
+        b.close()
+        for lat in lats:
+            lat.close()
+    errs_l = {}
+    for name in ("f16s", "bf16s"):
+        errs_l[name] = member_errors(specs, finals_l[name], finals_l["f32"])
+        log(f"learned mix {name} vs f32: max rel L2 {errs_l[name]['max']:.3e}, median {errs_l[name]['median']:.3e}")
+    res["learned_error_vs_f32"] = errs_l
+    model.close()
     re_meas = []
     for k, m in enumerate(specs):
         fluid = m["geometry"].reshape(-1) < 1

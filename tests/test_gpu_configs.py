@@ -5,6 +5,9 @@ steps): the flow is uniform in x, so the FP64 oracle on a 4-column channel
 computes every column of the 256-column one.  Bars:
   * FP64 storage: every node bit-identical to the oracle's column;
   * FP32 storage: max |u - u_oracle| <= 2e-3 max|u|;
+  * FP16S / BF16S storage (stochastic rounding): max |u - u_oracle| <=
+    2e-3 / 2e-2 max|u| (measured 8.8e-4 / 9.3e-3; with rounding to nearest
+    the 1e-6 body force is lost in the 16-bit slots and the flow stalls);
   * both against the analytic start-up series
       u(eta,t) = sum_{n odd} 4 g H^2/(nu pi^3 n^3) sin(n pi eta/H) (1 - exp(-n^2 pi^2 nu t/H^2))
     with walls halfway between the solid and fluid rows (H = 62,
@@ -47,24 +50,27 @@ def test_c1_poiseuille_channel():
     umax = np.max(np.abs(uo))
     dev_o = np.max(np.abs(uo[fluid] - ua[fluid]))
     case = cases.channel_d2q9(Lx=Lx, Ly=Ly, tau=tau, g=g)
-    for storage in (lbmc.FP64, lbmc.FP32):
+    tol = {lbmc.FP64: 0.0, lbmc.FP32: 2e-3, lbmc.FP16S: 2e-3, lbmc.BF16S: 2e-2}
+    for storage in (lbmc.FP64, lbmc.FP32, lbmc.FP16S, lbmc.BF16S):
         lat = cases.build(lbmc.Lattice, case, storage=storage)
         lat.run(0, T, T)
         rho, u = lat.get_macros()
         ux = u[0].reshape(Ly, Lx)
         f = lat.get_f().reshape(9, Ly, Lx)
-        # uniform in x
-        assert np.max(np.abs(ux - ux[:, :1])) <= (0 if storage == lbmc.FP64 else 1e-9)
-        dev = np.max(np.abs(ux[fluid, 0] - ua[fluid]))
-        err = np.max(np.abs(ux[:, 0] - uo))
-        print(f"C1 {lbmc.STORAGE_NAMES[storage]}: max|u - oracle| {err:.3e}, "
+        # uniform in x (to rounding; the 16-bit formats round stochastically)
+        xs = np.max(np.abs(ux - ux[:, :1]))
+        dev = np.max(np.abs(ux[fluid, :] - ua[fluid, None]))
+        err = np.max(np.abs(ux - uo[:, None]))
+        print(f"C1 {lbmc.STORAGE_NAMES[storage]}: max|u - oracle| {err:.3e}, max x-variation {xs:.3e}, "
               f"max|u - analytic| {dev:.3e} (oracle {dev_o:.3e}), u_max {umax:.4e}", flush=True)
         if storage == lbmc.FP64:
+            assert xs == 0
             assert np.array_equal(f, np.repeat(fo[:, :, None], Lx, axis=2))
             assert np.array_equal(ux, np.repeat(uo[:, None], Lx, axis=1))
         else:
-            assert err <= 2e-3 * umax
-        assert dev <= dev_o + 2e-3 * umax
+            assert xs <= (1e-9 if storage == lbmc.FP32 else 2 * tol[storage] * umax)
+            assert err <= tol[storage] * umax
+        assert dev <= dev_o + max(tol[storage], 2e-3) * umax
         lat.close()
 
 
