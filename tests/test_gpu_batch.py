@@ -118,6 +118,22 @@ def test_launch_shapes_match_oracle(shape):
         assert np.array_equal(fg, fr), c["name"]
 
 
+def test_batch_set_shape_rejects_bad_arguments():
+    lats = [cases.build(lbmc.Lattice, c, storage=lbmc.FP32) for c in _members(lbmc.D2Q9)[:2]]
+    b = lbmc.Batch(lats)
+    for shape in [(0, 256, 0), (8, 100, 0), (8, 0, 0), (8, 256 * 65, 0), (8, 256, -1)]:
+        with pytest.raises(lbmc.LbmcError) as e:
+            b.set_shape(*shape)
+        assert e.value.status == 1, shape
+    b.set_shape(3, 768, 4)
+    info = b.info()
+    assert info["chunks"] == sum(-(-lat.n // 768) for lat in lats)
+    assert b.shape() == (min(3, info["chunks"]), 768, 4)  # the grid never exceeds the chunks
+    b.close()
+    for lat in lats:
+        lat.close()
+
+
 def test_batch_rejects_mixed_members():
     a = lbmc.Lattice(lbmc.D2Q9, (8, 8), lbmc.BGK, tau=0.8, storage=lbmc.FP32)
     b = lbmc.Lattice(lbmc.D2Q9, (8, 8), lbmc.BGK, tau=0.8, storage=lbmc.FP64)

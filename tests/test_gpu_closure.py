@@ -298,5 +298,12 @@ def test_closure_rejects_unsupported():
     lat = lbmc.Lattice(lbmc.D2Q9, (8, 8), lbmc.BGK, tau=0.8, storage=lbmc.FP32)
     with pytest.raises(lbmc.LbmcError):
         lat.set_closure(m, 0.4, 2.0)
+    for bad in (float("nan"), float("inf")):
+        with pytest.raises(lbmc.LbmcError) as e:
+            lat.set_closure_shift(bad)
+        assert e.value.status == 1
+    for bad in (0, 4097):
+        with pytest.raises(lbmc.LbmcError):
+            lat.set_closure_interval(bad)
     lat.close()
     m.close()
