@@ -79,8 +79,11 @@ struct StBF16S {
 };
 
 // Random words of the stochastic stores: a hash of (node, call seed), then
-// one step of a 32-bit linear congruential generator per population (the
-// conversions use the high bits, the LCG's good ones).
+// one Weyl step (+ 2^32 / golden ratio) per population.  Each word is
+// uniform over the node hash; within a node the directions' words are
+// spread evenly over the interval (low discrepancy), so their rounding
+// decisions stay unbiased one by one and the node's total rounding error
+// is, if anything, smaller than with independent words.
 __device__ __forceinline__ unsigned sr_seed(int n, unsigned tseed)
 {
     unsigned h = (unsigned)n * 0x9E3779B1u ^ tseed * 0x85EBCA77u;
@@ -92,6 +95,6 @@ __device__ __forceinline__ unsigned sr_seed(int n, unsigned tseed)
     return h;
 }
 
-__device__ __forceinline__ unsigned sr_next(unsigned h) { return h * 1664525u + 1013904223u; }
+__device__ __forceinline__ unsigned sr_next(unsigned h) { return h + 0x9E3779B9u; }
 
 }  // namespace lbmc
