@@ -3,9 +3,10 @@
 turbulence: trained a priori on a resolved run, tested a posteriori on a
 coarse run from another initial field.
 
-  1. resolved runs on the fine grid (D3Q15 BGK, tau_f, FP32 storage) from
-     the seeded solenoidal field (E(k) ~ k^4 exp(-2 (k/k0)^2)); one seed for
-     training, one for testing;
+  1. resolved runs on the fine grid (D3Q15 BGK, tau_f) from the seeded
+     solenoidal field (E(k) ~ k^4 exp(-2 (k/k0)^2)); one seed for training
+     (FP32 storage), one for testing (FP64 storage by default: the kernels
+     that match the FP64 CPU restatement of the reference bit for bit);
   2. training data from the training run's snapshots (closure.sgs_targets,
      closure.features), subsampled; the network is trained on the GPU by the
      library's kernels;
@@ -90,6 +91,8 @@ def main():
                     help="further Smagorinsky constants run as baselines")
     ap.add_argument("--fine-c-smag", type=float, default=0.0,
                     help="Smagorinsky constant of the fine runs (0: plain BGK)")
+    ap.add_argument("--ref-storage", choices=["f64", "f32"], default="f64",
+                    help="population storage of the resolved test (reference) run")
     ap.add_argument("--out", type=str, default="closure_experiment.json")
     args = ap.parse_args()
 
@@ -146,7 +149,9 @@ def main():
     log(f"trained on {X.shape[0]} samples: loss {hist[0]:.4f} -> {hist[-1]:.4f} (normalised target variance 1)")
 
     # ---- 3. resolved test run: filtered reference spectra and the coarse initial field
-    lat, (ux, uy, uz) = fine_lattice(Lf, args.tau_fine, args.test_seed, args.k0, args.u_rms, lbmc.FP32,
+    ref_storage = lbmc.FP64 if args.ref_storage == "f64" else lbmc.FP32
+    t0 = time.perf_counter()
+    lat, (ux, uy, uz) = fine_lattice(Lf, args.tau_fine, args.test_seed, args.k0, args.u_rms, ref_storage,
                                      args.fine_c_smag)
     ub0 = np.stack([closure.box_filter(v, r) for v in (ux, uy, uz)])
     ref = {}
@@ -158,6 +163,8 @@ def main():
         ref[ts] = spectrum(np.stack([closure.box_filter(u[a], r) for a in range(3)]))
         log(f"reference at fine step {ts}: E = {ref[ts].sum():.5g}")
     lat.close()
+    res["ref_storage"] = args.ref_storage
+    res["fine_ref_seconds"] = round(time.perf_counter() - t0, 1)
     del u
 
     # ---- 4. coarse runs
