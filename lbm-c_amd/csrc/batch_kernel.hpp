@@ -175,7 +175,6 @@ __global__ __launch_bounds__(256) void batch_advance_kernel(BatchMember<C> *memb
             m.store_count = sc;
             m.a.store = sc == 0 ? 1 : 0;
         }
-        m.a.tseed += 1u;
         if (m.closure_every > 1) {
             int ph = m.closure_phase + 1;
             if (ph == m.closure_every) ph = 0;

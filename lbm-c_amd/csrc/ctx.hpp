@@ -116,7 +116,6 @@ inline void fill_args(const lbmc_ctx *c, StepArgs<C> &a, long long t, int store)
     a.c_smag = (C)c->desc.c_smag;
     for (int i = 0; i < 15; i++) a.tau_mrt[i] = (C)c->desc.tau_mrt[i];
     a.store = store ? 1 : 0;
-    a.tseed = (unsigned)t;
     a.porous = c->porous ? 1 : 0;
     a.quirks = c->desc.quirks;
     a.mlp = c->mlp_img;

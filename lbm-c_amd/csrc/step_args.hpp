@@ -69,7 +69,6 @@ struct StepArgs {
     C c_smag;
     C tau_mrt[15];
     int store;               // store rho/u this call
-    unsigned tseed;          // per-call seed of the stochastic stores (the call index)
     int porous;              // NTPOR/MRTPOR with porous nodes present
     unsigned quirks;
     const void *mlp;         // closure weights (MlpImage) when the closure is on

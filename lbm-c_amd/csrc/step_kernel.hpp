@@ -326,7 +326,13 @@ __device__ __forceinline__ void scatter_node(const StepArgs<typename St::C> &a, 
     using S = typename St::S;
     S *const fb = (S *)a.f;
     unsigned r = 0;
-    if constexpr (St::stochastic) r = sr_seed(n, a.tseed);
+    if constexpr (St::stochastic) {
+        // the call's seed from its shift counters s_d = t mod L_d: one value
+        // per t mod lcm(L), so a graph replay, a direct launch and a batch
+        // member draw the same words for the same call
+        const unsigned ts = (unsigned)a.s[0] + (unsigned)a.L[0] * ((unsigned)a.s[1] + (unsigned)a.L[1] * (unsigned)a.s[2]);
+        r = sr_seed(n, ts);
+    }
     sfor<0, L::Q>([&](auto ii) {
         constexpr int i = decltype(ii)::value;
         const int k = n + dd[0][L::ex[i] + 1] + dd[1][L::ey[i] + 1] + dd[2][L::ez[i] + 1];

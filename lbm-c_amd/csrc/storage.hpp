@@ -78,7 +78,8 @@ struct StBF16S {
     }
 };
 
-// Random words of the stochastic stores: a hash of (node, call seed), then
+// Random words of the stochastic stores: a hash of (node, call seed: the
+// call's shift counters), then
 // one Weyl step (+ 2^32 / golden ratio) per population.  Each word is
 // uniform over the node hash; within a node the directions' words are
 // spread evenly over the interval (low discrepancy), so their rounding

@@ -38,7 +38,7 @@ def decaying_turbulence(L, k0=4.0, u_rms=0.05, seed=0):
     # shell spectrum E(k) ~ k^4 exp(-2(k/k0)^2): amplitude ~ sqrt(E(k)/k^2)
     with np.errstate(invalid="ignore", divide="ignore"):
         amp = np.where(k > 0, np.sqrt(k ** 4 * np.exp(-2.0 * (k / k0) ** 2) / np.where(k2 > 0, k2, 1.0)), 0.0)
-    u = [np.fft.irfftn(c * amp, s=shape) for c in comps]
+    u = [np.fft.irfftn(c * amp, s=shape, axes=(0, 1, 2)) for c in comps]
     scale = u_rms / np.sqrt(np.mean(u[0] ** 2 + u[1] ** 2 + u[2] ** 2) / 3.0)
     return tuple(np.ascontiguousarray(v * scale) for v in u)
 

@@ -60,13 +60,17 @@ def test_batch_fp64_matches_oracle_bitwise(lattice, t0):
         assert np.array_equal(ug, ur), c["name"]
 
 
-def test_batch_fp32_matches_single_lattices():
+@pytest.mark.parametrize("storage", [lbmc.FP32, lbmc.FP16S, lbmc.BF16S], ids=["fp32", "fp16s", "bf16s"])
+def test_batch_matches_single_lattices(storage):
+    """Same node arithmetic and, for the 16-bit formats, the same stochastic
+    rounding words (seeded by node and the call's shift counters): a batch
+    ends where its members end on their own."""
     group = _members(lbmc.D2Q9)
-    print(f"[batch] FP32 members={len(group)}", flush=True)
+    print(f"[batch] storage={storage} members={len(group)}", flush=True)
     steps = 70
-    got = _run_batch(group, lbmc.FP32, 0, steps, store_every=7)
+    got = _run_batch(group, storage, 0, steps, store_every=7)
     for c, (fg, (rg, ug)) in zip(group, got):
-        lat = cases.build(lbmc.Lattice, c, storage=lbmc.FP32)
+        lat = cases.build(lbmc.Lattice, c, storage=storage)
         lat.run(0, steps, 7)
         fs = lat.get_f()
         rs, us = lat.get_macros()

@@ -103,7 +103,7 @@ def test_c2_porous_media_full_size():
             assert np.max(np.abs(u - uo)) <= K
 
 
-@pytest.mark.parametrize("storage", [0, 1], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("storage", [0, 1, 2, 3], ids=["fp64", "fp32", "fp16s", "bf16s"])
 def test_graph_replay_matches_direct(storage, monkeypatch):
     case = cases.pressure_d2q9(Lx=24, Ly=12, collision=cases.MRT, steps=0)
     period = 24  # lcm(24, 12); store every 6 keeps the period
