@@ -25,7 +25,7 @@ __device__ __forceinline__ void closure_update(const StepArgs<float> &a, int n, 
                                                const MlpInfer &w, mlp_v8b *xw)
 {
     constexpr int D = L::D;
-    const unsigned m = ldg(a.mask + n);
+    const unsigned m = a.all_fluid ? 0u : ldg(a.mask + n);
     float F[3] = {0.f, 0.f, 0.f};
     if constexpr (GUO) {
         sfor<0, D>([&](auto dd) {
@@ -63,7 +63,7 @@ __device__ __forceinline__ void closure_update(const StepArgs<float> &a, int n, 
 template <class L, class St, bool GUO>
 __device__ __forceinline__ void closure_reuse_node(const StepArgs<float> &a, int n, float (&v)[L::Q])
 {
-    if (ldg(a.mask + n) == 0) {
+    if (a.all_fluid || ldg(a.mask + n) == 0) {
         constexpr int D = L::D;
         float F[3] = {0.f, 0.f, 0.f};
         if constexpr (GUO) {

@@ -38,6 +38,7 @@ struct lbmc_ctx {
 
     bool fields_set = false;
     bool porous = false;
+    bool all_fluid = false;  // every node plain fluid (mask all 0): the step skips the mask
     double F[3] = {0, 0, 0};
     int ncls = 1;
     uint16_t cls_host[256] = {};
@@ -117,6 +118,7 @@ inline void fill_args(const lbmc_ctx *c, StepArgs<C> &a, long long t, int store)
     for (int i = 0; i < 15; i++) a.tau_mrt[i] = (C)c->desc.tau_mrt[i];
     a.store = store ? 1 : 0;
     a.porous = c->porous ? 1 : 0;
+    a.all_fluid = c->all_fluid ? 1 : 0;
     a.quirks = c->desc.quirks;
     a.mlp = c->mlp_img;
     a.tcache = c->tcache;

@@ -442,7 +442,12 @@ lbmc_status lbmc_set_fields(lbmc_ctx *c, const double *geometry, const int *macr
     }
     cls_of[0] = 0;
     std::vector<uint8_t> maskv((size_t)N);
-    for (long long n = 0; n < N; n++) maskv[(size_t)n] = (uint8_t)cls_of[descv[(size_t)n]];
+    bool all_fluid = true;
+    for (long long n = 0; n < N; n++) {
+        maskv[(size_t)n] = (uint8_t)cls_of[descv[(size_t)n]];
+        all_fluid = all_fluid && maskv[(size_t)n] == 0;
+    }
+    c->all_fluid = all_fluid;
     HIP_TRY(c, hipMemcpy(c->mask, maskv.data(), (size_t)N, hipMemcpyHostToDevice), "mask upload");
     HIP_TRY(c, hipMemcpy(c->cls, table, sizeof(table), hipMemcpyHostToDevice), "class upload");
     memcpy(c->cls_host, table, sizeof(table));

@@ -70,6 +70,7 @@ struct StepArgs {
     C tau_mrt[15];
     int store;               // store rho/u this call
     int porous;              // NTPOR/MRTPOR with porous nodes present
+    int all_fluid;           // every node is plain fluid: class 0 without reading the mask
     unsigned quirks;
     const void *mlp;         // closure weights (MlpImage) when the closure is on
     float *tcache;           // closure: per-node tau of the last evaluation (NaN: fall back)

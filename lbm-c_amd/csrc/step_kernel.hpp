@@ -47,7 +47,7 @@ __device__ __forceinline__ void node_update(const StepArgs<typename St::C> &a, i
     // FP32 compute: plain fluid nodes of the BGK family take the regrouped
     // update (fast_bgk.hpp); everything else follows the reference's order.
     constexpr bool FAST = std::is_same<C, float>::value;
-    const unsigned m = ldg(a.mask + n);
+    const unsigned m = a.all_fluid ? 0u : ldg(a.mask + n);
     if constexpr (FAST && FAM == 0) {
         if (m == 0) {
             C F[3] = {C(0), C(0), C(0)};
