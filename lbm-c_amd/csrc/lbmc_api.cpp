@@ -132,19 +132,21 @@ lbmc_status launch_step(lbmc_ctx *c, long long t, int store)
     return LBMC_OK;
 }
 
-// Resident blocks per CU of the step kernels (0: no cap), measured on
-// 256^3 D3Q15 lattices: FP32 storage 3 (BGK family) or 4 (MRT family),
-// FP16S/BF16S 5, FP64 no gain.  Only lattices well beyond the 256 MB
-// last-level cache are capped: a cache-resident lattice is latency-bound and
-// wants every wave it can get.
+// Resident blocks per CU of the step kernels (0: no cap), the best of
+// sweeps over 0/3/4/5/6/8 on 256^3 D3Q15 and 4096^2 D2Q9 lattices:
+// D3Q15 FP32 3 (BGK family) or 4 (MRT family), FP16S/BF16S 5; D2Q9 FP32 4,
+// FP16S/BF16S none; FP64 no gain either way.  Only lattices well beyond the
+// 256 MB last-level cache are capped: a cache-resident lattice is
+// latency-bound and wants every wave it can get.
 int default_blocks_per_cu(const lbmc_ctx *c)
 {
     const double pop_bytes = (double)c->sbytes * c->Q * c->npad;
     if (pop_bytes < 512.0 * (1 << 20)) return 0;
+    const bool d3 = c->desc.lattice == LBMC_D3Q15;
     switch (c->storage) {
-    case LBMC_FP32: return c->key.family == 0 ? 3 : 4;
+    case LBMC_FP32: return d3 ? (c->key.family == 0 ? 3 : 4) : 4;
     case LBMC_FP16S:
-    case LBMC_BF16S: return 5;
+    case LBMC_BF16S: return d3 ? 5 : 0;
     default: return 0;
     }
 }
