@@ -146,7 +146,8 @@ def main():
                 "MLUPS": round(N * vsteps / (vms * 1e-3) / 1e6, 1),
                 "ms_per_step": round(vms / vsteps, 5),
                 "bytes_per_node": vb,
-                "achieved_GBs": round(vb * N / (vms / vsteps * 1e-3) / 1e9, 1)}
+                "achieved_GBs": round(vb * N / (vms / vsteps * 1e-3) / 1e9, 1),
+                "frac": round(vb * N / (vms / vsteps * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
             v.close()
             log(f"bench: {lbmc.STORAGE_NAMES[st]} {variants[lbmc.STORAGE_NAMES[st]]}")
 
@@ -217,7 +218,8 @@ def main():
                 lt.close()
             return {"MLUPS": round(EN * esteps / (ems * 1e-3) / 1e6, 1), "ms_per_step": round(ems / esteps, 5),
                     "grid": grid, "chunk_nodes": chunk_nodes, "blocks_per_cu": bpc,
-                    "achieved_GBs": round(73 * EN / (ems / esteps * 1e-3) / 1e9, 1)}
+                    "achieved_GBs": round(73 * EN / (ems / esteps * 1e-3) / 1e9, 1),
+                    "frac": round(73 * EN / (ems / esteps * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
         ensemble_line = {"members": 64, "nodes": EN, "bytes_per_node": 2 * 9 * 4 + 1,
                          "members_kind": "MRT or BGK + learned closure",
