@@ -396,10 +396,7 @@ lbmc_status lbmc_batch_autotune(lbmc_batch *b, long long t0, int steps_per_trial
 {
     if (!b || steps_per_trial <= 0 || t0 < 0) return bfail(b, LBMC_ERR_INVALID, "lbmc_batch_autotune: bad argument");
     B_HIP(b, hipSetDevice(b->device));
-    int ncu = 256;
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, b->device) == hipSuccess && prop.multiProcessorCount > 0)
-        ncu = prop.multiProcessorCount;
+    const int ncu = b->ncu;
     struct Cand {
         int sub, grid_per_cu, cap;  // grid_per_cu 0: one block per chunk
     };
