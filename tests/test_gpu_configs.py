@@ -16,6 +16,8 @@ computes every column of the 256-column one.  Bars:
 Graph replay: lbmc_run replays a captured graph of lcm(Lx, Ly, Lz) calls;
 it must give the same bits as launching every call.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -119,3 +121,47 @@ def test_graph_replay_matches_direct(storage, monkeypatch):
     assert np.array_equal(d.get_f(), g.get_f())
     assert np.array_equal(d.get_macros()[0], g.get_macros()[0])
     assert np.array_equal(d.get_macros()[1], g.get_macros()[1])
+
+
+def test_c3_turbulence_full_size():
+    """D3Q15 BGK + Smagorinsky (C_s 0.17, tau 0.51) at 256^3 from the bench's
+    seeded turbulence field, 3 calls (call 0 collides in place, calls 1-2
+    stream and collide) against the oracle: FP64 storage bit-identical
+    (the all-fluid path, which skips the mask read); velocity within 1e-5 /
+    2e-3 / 2e-2 of max|u| at FP32 / FP16S / BF16S storage."""
+    import workloads
+
+    L, steps = 256, 3
+    n = L ** 3
+    print("[c3] field and oracle 256^3, 3 calls", flush=True)
+    ux, uy, uz = workloads.decaying_turbulence(L, k0=4.0, u_rms=0.05, seed=2024)
+    u0 = [ux.reshape(-1), uy.reshape(-1), uz.reshape(-1)]
+    del ux, uy, uz
+    desc = dict(lattice=lbmc.D3Q15, length=(L, L, L), collision_type=lbmc.BGK, tau=0.51, c_smag=0.17)
+    pyoracle.set_threads(min(16, os.cpu_count() or 1))
+    o = pyoracle.Oracle(**desc)
+    o.set_fields(geometry=np.zeros(n))
+    o.init_equilibrium(np.ones(n), u0)
+    o.run(0, steps, steps)
+    fo = o.get_f()
+    ro, uo = o.get_macros()
+    o.close()
+    gmax = None
+    for storage in (lbmc.FP64, lbmc.FP32, lbmc.FP16S, lbmc.BF16S):
+        print(f"[c3] GPU storage {storage}", flush=True)
+        lat = lbmc.Lattice(**desc, storage=storage)
+        lat.set_fields(geometry=np.zeros(n))
+        lat.init_equilibrium(np.ones(n), u0)
+        lat.run(0, steps, steps)
+        f = lat.get_f()
+        rho, u = lat.get_macros()
+        lat.close()
+        if storage == lbmc.FP64:
+            assert np.array_equal(f, fo)
+            assert np.array_equal(rho, ro) and np.array_equal(u, uo)
+            gmax = float(np.max(np.abs(u)))
+            continue
+        du = float(np.max(np.abs(u - uo)))
+        print(f"[c3] {lbmc.STORAGE_NAMES[storage]}: max|u - u_oracle| {du:.3e} (max|u| {gmax:.3e})", flush=True)
+        tol = {lbmc.FP32: 1e-5, lbmc.FP16S: 2e-3, lbmc.BF16S: 2e-2}[storage]
+        assert du <= tol * gmax
