@@ -165,3 +165,47 @@ def test_c3_turbulence_full_size():
         print(f"[c3] {lbmc.STORAGE_NAMES[storage]}: max|u - u_oracle| {du:.3e} (max|u| {gmax:.3e})", flush=True)
         tol = {lbmc.FP32: 1e-5, lbmc.FP16S: 2e-3, lbmc.BF16S: 2e-2}[storage]
         assert du <= tol * gmax
+
+
+def test_c5_ensemble_full_size():
+    """The 64-member D2Q9 ensemble (11.45 M nodes; MRT and BGK members with
+    Guo forcing, seeded discs) with FP64 storage in one batch, 5 calls from
+    the static start with the autotuned launch shape: every member
+    bit-identical to the oracle."""
+    import workloads
+
+    specs = workloads.ensemble_d2q9(64, 0)
+    print(f"[c5] 64 members, {sum(m['length'][0] * m['length'][1] for m in specs)} nodes", flush=True)
+    pyoracle.set_threads(min(16, os.cpu_count() or 1))
+
+    def build(cls, m, **extra):
+        Lx, Ly = m["length"]
+        n = Lx * Ly
+        coll = cases.MRT if m["collision"] == "MRT" else cases.BGK
+        lat = cls(lattice=cases.D2Q9, length=(Lx, Ly), collision_type=coll, forcing=True, tau=m["tau"],
+                  tau_mrt=m["rates"] if m["rates"] else cases.d2q9_rates(m["tau"]), **extra)
+        lat.set_fields(geometry=m["geometry"].reshape(-1), force=[np.full(n, m["force"]), np.zeros(n)])
+        lat.init_static()
+        return lat
+
+    lats = [build(lbmc.Lattice, m, storage=lbmc.FP64) for m in specs]
+    b = lbmc.Batch(lats)
+    b.autotune(0, steps_per_trial=2)
+    shape = b.shape()
+    b.close()
+    for lat in lats:
+        lat.init_static()
+    b = lbmc.Batch(lats)
+    b.set_shape(*shape)
+    b.run(0, 5, 5)
+    b.close()
+    for k, (m, lat) in enumerate(zip(specs, lats)):
+        f = lat.get_f()
+        rho, u = lat.get_macros()
+        lat.close()
+        o = build(pyoracle.Oracle, m)
+        o.run(0, 5, 5)
+        assert np.array_equal(f, o.get_f()), k
+        ro, uo = o.get_macros()
+        assert np.array_equal(rho, ro) and np.array_equal(u, uo), k
+        o.close()
