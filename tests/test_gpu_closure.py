@@ -379,3 +379,14 @@ def test_trained_closure_beats_smagorinsky_a_posteriori():
     print(f"[closure] spectrum error: Smagorinsky {smag}, trained closure {learned}", flush=True)
     for es, el in zip(smag, learned):
         assert np.isfinite(el) and el < 0.5 * es
+
+
+def test_closure_c4_coarse_full_size():
+    """The closure step at the coarse size of the closure study (128^3,
+    tau_c = 0.50125, Smagorinsky fallback C_s 0.17), 2 calls, against the
+    FP64 closure oracle: network output inside the band at every node."""
+    print("[closure] 128^3, 2 calls", flush=True)
+    (rd, ud), (ro, uo), fb, edge = _run_pair(lbmc.D3Q15, (128, 128, 128), 0.50125, 0.17, lbmc.FP32, 2)
+    assert fb == 0.0 and edge > 1e-3
+    assert np.max(np.abs(ud - uo)) < 1e-5
+    assert np.max(np.abs(rd - ro)) < 1e-5
