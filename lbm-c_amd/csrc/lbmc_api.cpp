@@ -133,9 +133,10 @@ lbmc_status launch_step(lbmc_ctx *c, long long t, int store)
 }
 
 // Resident blocks per CU of the step kernels (0: no cap), the best of
-// sweeps over 0/3/4/5/6/8 on 256^3 D3Q15 and 4096^2 D2Q9 lattices:
-// D3Q15 FP32 3 (BGK family) or 4 (MRT family), FP16S/BF16S 5; D2Q9 FP32 4,
-// FP16S/BF16S none; FP64 no gain either way.  Only lattices well beyond the
+// sweeps over 0/2/3/4/5/6/8 on 256^3 D3Q15 and 4096^2 D2Q9 lattices:
+// D3Q15 FP32 3 (BGK family) or 4 (MRT family); D2Q9 FP32 4; FP16S/BF16S
+// none (their loads run at raised wave priority instead, and a cap of 5 then
+// costs 2.5 %); FP64 no gain either way.  Only lattices well beyond the
 // 256 MB last-level cache are capped: a cache-resident lattice is
 // latency-bound and wants every wave it can get.
 int default_blocks_per_cu(const lbmc_ctx *c)
@@ -145,8 +146,6 @@ int default_blocks_per_cu(const lbmc_ctx *c)
     const bool d3 = c->desc.lattice == LBMC_D3Q15;
     switch (c->storage) {
     case LBMC_FP32: return d3 ? (c->key.family == 0 ? 3 : 4) : 4;
-    case LBMC_FP16S:
-    case LBMC_BF16S: return d3 ? 5 : 0;
     default: return 0;
     }
 }

@@ -357,7 +357,14 @@ template <class L, class St, int FAM, bool GUO, bool SMAG, bool MFIX, bool WIDE>
 __device__ __forceinline__ void step_node(const StepArgs<typename St::C> &a, int n, const int (&dd)[3][3])
 {
     typename St::C v[L::Q];
+    constexpr bool PRIO = St::gather_prio && L::D == 3;
+    if constexpr (PRIO) __builtin_amdgcn_s_setprio(1);
     gather_node<L, St, WIDE>(a, n, dd, v);
+    if constexpr (PRIO) {
+        // keep the loads above the priority drop
+        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_setprio(0);
+    }
     node_update<L, St, FAM, GUO, SMAG, MFIX>(a, n, v);
     scatter_node<L, St, WIDE>(a, n, dd, v);
 }
