@@ -15,6 +15,15 @@ namespace lbmc {
 template <class L, class St, int FAM, bool GUO, bool SMAG, bool MFIX>
 static hipError_t launch_one(const StepArgs<typename St::C> &a, int grid, hipStream_t s, bool wide, int lds)
 {
+    if constexpr (sizeof(typename St::S) == 2) {
+        // even x shift on an even-length row: two nodes per lane
+        if (!wide && !a.remap && (a.L[0] & 1) == 0 && (a.s[0] & 1) == 0) {
+            const int pgrid = (a.nnodes / 2 + kBlock - 1) / kBlock;
+            hipLaunchKernelGGL((step_pair_kernel<L, St, FAM, GUO, SMAG, MFIX>), dim3(pgrid), dim3(kBlock), lds, s,
+                               a);
+            return hipGetLastError();
+        }
+    }
     if (wide)
         hipLaunchKernelGGL((step_kernel<L, St, FAM, GUO, SMAG, MFIX, true>), dim3(grid), dim3(kBlock), lds, s, a);
     else

@@ -79,6 +79,29 @@ def test_batch_matches_single_lattices(storage):
         assert np.max(np.abs(ug - us)) <= 1e-6, c["name"]
 
 
+@pytest.mark.parametrize("t0", [0, 2 ** 31 + 5])
+@pytest.mark.parametrize("storage", [lbmc.FP16S, lbmc.BF16S], ids=["fp16s", "bf16s"])
+@pytest.mark.parametrize("lattice", [lbmc.D2Q9, lbmc.D3Q15], ids=["d2q9", "d3q15"])
+def test_pair_steps_match_one_node_steps_bitwise(lattice, storage, t0):
+    """A 16-bit lattice stepped on its own takes two nodes per lane on calls
+    with an even x shift (every case here has an even Lx); a batch member
+    takes one node per lane on every call, with the same per-node update and
+    stochastic-rounding words: populations and macros bit-identical."""
+    group = _members(lattice)
+    print(f"[pairs] lattice={lattice} storage={storage} t0={t0} members={len(group)}", flush=True)
+    assert all(c["desc"]["length"][0] % 2 == 0 for c in group)
+    steps = 21
+    got = _run_batch(group, storage, t0, steps, store_every=7)
+    for c, (fg, (rg, ug)) in zip(group, got):
+        lat = cases.build(lbmc.Lattice, c, storage=storage)
+        lat.run(t0, steps, 7)
+        fs = lat.get_f()
+        rs, us = lat.get_macros()
+        lat.close()
+        assert np.array_equal(fg, fs), c["name"]
+        assert np.array_equal(rg, rs) and np.array_equal(ug, us), c["name"]
+
+
 def test_batch_store_schedule():
     # store_every = 10 over 25 calls: the stored macros are those of call 19
     group = _members(lbmc.D2Q9)[:3]
