@@ -133,7 +133,7 @@ lbmc_status launch_step(lbmc_ctx *c, long long t, int store)
 }
 
 // Resident blocks per CU of the step kernels (0: no cap), the best of
-// sweeps over 0/2/3/4/5/6/8 on 256^3 D3Q15 and 4096^2 D2Q9 lattices:
+// sweeps over 0/3/4/5/6/8 on 256^3 D3Q15 and 4096^2 D2Q9 lattices:
 // D3Q15 FP32 3 (BGK family) or 4 (MRT family); D2Q9 FP32 4; FP16S/BF16S
 // none (their loads run at raised wave priority instead, and a cap of 5 then
 // costs 2.5 %); FP64 no gain either way.  Only lattices well beyond the
@@ -326,6 +326,10 @@ lbmc_status lbmc_create(const lbmc_desc *desc, lbmc_ctx **out)
         // (reserved LDS, which the step kernels do not use) raises it.
         int bpc = default_blocks_per_cu(c);
         if (const char *env = getenv("LBMC_STEP_BLOCKS_PER_CU")) bpc = atoi(env);
+        // two nodes per lane on calls with an even x shift (step_pair_kernel);
+        // LBMC_STEP_PAIRS=0 keeps one node per lane on every call
+        c->key.pairs = 1;
+        if (const char *env = getenv("LBMC_STEP_PAIRS")) c->key.pairs = atoi(env) != 0;
         int lds_cu = 0;
         if ((e = hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
                                        desc->device)) != hipSuccess)

@@ -33,6 +33,7 @@ struct KernelKey {
     int closure;  // learned relaxation-time closure (BGK family, FP32 compute)
     int lds;      // dynamic LDS bytes per block of the step kernels (which use
                   //   none): caps the blocks resident per CU, 0 = no cap
+    int pairs;    // two nodes per lane on calls with an even x shift
 };
 
 enum : unsigned {

@@ -424,14 +424,16 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const StepArgs<typename St
     step_node<L, St, FAM, GUO, SMAG, MFIX, WIDE>(a, n, dd);
 }
 
-// Two nodes per lane for the 16-bit formats on calls whose x shift s_x =
-// t mod Lx is even, with Lx even.  Nodes n = 2p and n + 1 lie in one row and
-// have the same slot offsets in every direction (a wrap between x and x + 1
-// would need x + s_x odd), and slot n + offset is even, so each direction's
-// two populations are one aligned 32-bit word: one load and one store per
-// direction for both nodes, half the memory instructions of step_kernel.
-// Each node takes step_kernel's update and stochastic-rounding words, so the
-// results are bit-identical.  Direction arrays below 4 GiB only.
+// Two nodes per lane for the 16-bit formats on calls whose x shift
+// s_x = t mod Lx is even, with Lx even.  Nodes n = 2p and n + 1 lie in one
+// row and have the same slot offsets in every direction (a wrap between x
+// and x + 1 would need x + s_x odd), and slot n + offset is even, so each
+// direction's two populations are one aligned 32-bit word: one load and one
+// store per direction for both nodes, half the memory instructions of
+// step_kernel.  Each node takes step_kernel's update and stochastic-rounding
+// words, so the results are bit-identical.  Direction arrays below 4 GiB
+// only.  (FP32 pairs in 64-bit words are correct too but 2-3 % slower than
+// one node per lane at every residency cap, so FP32 keeps step_kernel.)
 template <class L, class St, int FAM, bool GUO, bool SMAG, bool MFIX>
 __global__ __launch_bounds__(kBlock) void step_pair_kernel(const StepArgs<float> a)
 {

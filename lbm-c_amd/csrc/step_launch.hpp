@@ -13,11 +13,12 @@ namespace lbmc {
 
 // lds: dynamic LDS bytes per block (caps the resident blocks per CU)
 template <class L, class St, int FAM, bool GUO, bool SMAG, bool MFIX>
-static hipError_t launch_one(const StepArgs<typename St::C> &a, int grid, hipStream_t s, bool wide, int lds)
+static hipError_t launch_one(const StepArgs<typename St::C> &a, int grid, hipStream_t s, bool wide, int lds,
+                             bool pairs)
 {
     if constexpr (sizeof(typename St::S) == 2) {
         // even x shift on an even-length row: two nodes per lane
-        if (!wide && !a.remap && (a.L[0] & 1) == 0 && (a.s[0] & 1) == 0) {
+        if (pairs && !wide && !a.remap && (a.L[0] & 1) == 0 && (a.s[0] & 1) == 0) {
             const int pgrid = (a.nnodes / 2 + kBlock - 1) / kBlock;
             hipLaunchKernelGGL((step_pair_kernel<L, St, FAM, GUO, SMAG, MFIX>), dim3(pgrid), dim3(kBlock), lds, s,
                                a);
@@ -66,20 +67,20 @@ static hipError_t launch_lattice(const KernelKey &k, const StepArgs<typename St:
     }
     if (k.family == 0) {
         if (k.guo) {
-            return k.smag ? launch_one<L, St, 0, true, true, false>(a, grid, s, k.wide != 0, k.lds)
-                          : launch_one<L, St, 0, true, false, false>(a, grid, s, k.wide != 0, k.lds);
+            return k.smag ? launch_one<L, St, 0, true, true, false>(a, grid, s, k.wide != 0, k.lds, k.pairs != 0)
+                          : launch_one<L, St, 0, true, false, false>(a, grid, s, k.wide != 0, k.lds, k.pairs != 0);
         }
-        return k.smag ? launch_one<L, St, 0, false, true, false>(a, grid, s, k.wide != 0, k.lds)
-                      : launch_one<L, St, 0, false, false, false>(a, grid, s, k.wide != 0, k.lds);
+        return k.smag ? launch_one<L, St, 0, false, true, false>(a, grid, s, k.wide != 0, k.lds, k.pairs != 0)
+                      : launch_one<L, St, 0, false, false, false>(a, grid, s, k.wide != 0, k.lds, k.pairs != 0);
     }
     if constexpr (L::D == 3) {
         if (k.mfix) {
-            return k.guo ? launch_one<L, St, 1, true, false, true>(a, grid, s, k.wide != 0, k.lds)
-                         : launch_one<L, St, 1, false, false, true>(a, grid, s, k.wide != 0, k.lds);
+            return k.guo ? launch_one<L, St, 1, true, false, true>(a, grid, s, k.wide != 0, k.lds, k.pairs != 0)
+                         : launch_one<L, St, 1, false, false, true>(a, grid, s, k.wide != 0, k.lds, k.pairs != 0);
         }
     }
-    return k.guo ? launch_one<L, St, 1, true, false, false>(a, grid, s, k.wide != 0, k.lds)
-                 : launch_one<L, St, 1, false, false, false>(a, grid, s, k.wide != 0, k.lds);
+    return k.guo ? launch_one<L, St, 1, true, false, false>(a, grid, s, k.wide != 0, k.lds, k.pairs != 0)
+                 : launch_one<L, St, 1, false, false, false>(a, grid, s, k.wide != 0, k.lds, k.pairs != 0);
 }
 
 template <class St>

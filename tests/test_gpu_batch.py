@@ -4,6 +4,8 @@ oracle with FP64 storage (each member a different collision code, boundary
 mix and size), and against the single-lattice kernels with FP32 storage;
 also from a large t0 (index-shift wrap), across graph replays, and with the
 autotuned grid."""
+import os
+
 import numpy as np
 import pytest
 
@@ -86,7 +88,9 @@ def test_pair_steps_match_one_node_steps_bitwise(lattice, storage, t0):
     """A 16-bit lattice stepped on its own takes two nodes per lane on calls
     with an even x shift (every case here has an even Lx); a batch member
     takes one node per lane on every call, with the same per-node update and
-    stochastic-rounding words: populations and macros bit-identical."""
+    stochastic-rounding words: populations and macros bit-identical.  So
+    is the same lattice with LBMC_STEP_PAIRS=0 (read at create: one node per
+    lane on every call)."""
     group = _members(lattice)
     print(f"[pairs] lattice={lattice} storage={storage} t0={t0} members={len(group)}", flush=True)
     assert all(c["desc"]["length"][0] % 2 == 0 for c in group)
@@ -100,6 +104,14 @@ def test_pair_steps_match_one_node_steps_bitwise(lattice, storage, t0):
         lat.close()
         assert np.array_equal(fg, fs), c["name"]
         assert np.array_equal(rg, rs) and np.array_equal(ug, us), c["name"]
+        os.environ["LBMC_STEP_PAIRS"] = "0"
+        try:
+            lat = cases.build(lbmc.Lattice, c, storage=storage)
+        finally:
+            del os.environ["LBMC_STEP_PAIRS"]
+        lat.run(t0, steps, 7)
+        assert np.array_equal(lat.get_f(), fs), c["name"]
+        lat.close()
 
 
 def test_batch_store_schedule():
