@@ -330,11 +330,14 @@ lbmc_status lbmc_create(const lbmc_desc *desc, lbmc_ctx **out)
         // LBMC_STEP_PAIRS=0 keeps one node per lane on every call
         c->key.pairs = 1;
         if (const char *env = getenv("LBMC_STEP_PAIRS")) c->key.pairs = atoi(env) != 0;
-        int lds_cu = 0;
+        int lds_cu = 0, lds_block = 0;
         if ((e = hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
-                                       desc->device)) != hipSuccess)
+                                       desc->device)) != hipSuccess ||
+            (e = hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, desc->device)) !=
+                hipSuccess)
             return bail(hip_fail(c, e, "device attribute"));
-        c->key.lds = (bpc >= 3 && bpc <= 16 && lds_cu > 0) ? lds_cu / bpc - 1024 : 0;
+        const int lds = (bpc >= 1 && bpc <= 16 && lds_cu > 0) ? lds_cu / bpc - 1024 : 0;
+        c->key.lds = lds <= lds_block ? lds : 0;
     }
     if ((e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess)
         return bail(hip_fail(c, e, "hipEventCreate"));
