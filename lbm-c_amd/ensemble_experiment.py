@@ -86,6 +86,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10000, help="steps of the error study")
     ap.add_argument("--time-steps", type=int, default=1000, help="timed steps of the throughput runs")
     ap.add_argument("--out", type=str, default="ensemble_experiment.json")
+    ap.add_argument("--plain-fp32-only", action="store_true",
+                    help="only the plain-mix FP32 throughput run (short; for kernel traces)")
     args = ap.parse_args()
 
     specs = workloads.ensemble_d2q9(args.count, args.seed)
@@ -123,6 +125,9 @@ def main():
         name = lbmc.STORAGE_NAMES[st]
         thr[name] = timed(st)
         log(f"plain {name}: {thr[name]}")
+        if args.plain_fp32_only:
+            model.close()
+            return
     res["throughput"] = thr
     thr_l = {}
     for st in (lbmc.FP32, lbmc.FP16S, lbmc.BF16S):
