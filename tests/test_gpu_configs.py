@@ -209,3 +209,48 @@ def test_c5_ensemble_full_size():
         ro, uo = o.get_macros()
         assert np.array_equal(rho, ro) and np.array_equal(u, uo), k
         o.close()
+
+
+def test_max_size_wide_offsets_match_tiled_lattice():
+    """A 1024^3 D3Q15 FP32 lattice (2^30 nodes: every direction array is
+    4 GiB, so the step takes its 64-bit-offset kernels; 64 GB of
+    populations) holding a field of period 64 evolves exactly as the 64^3
+    lattice holding one period (periodic boxes, node-local arithmetic): the
+    stored macros of sampled nodes are bit-identical after 4 calls, and the
+    kinetic energy is 4096 times the small lattice's to summation rounding."""
+    import workloads
+
+    P, L, steps = 64, 1024, 4
+    desc = dict(collision_type=lbmc.BGK, tau=0.51, c_smag=0.17, storage=lbmc.FP32)
+    u_small = workloads.decaying_turbulence(P, k0=4.0, u_rms=0.05, seed=7)
+    print("[max size] 64^3 lattice", flush=True)
+    small = lbmc.Lattice(lbmc.D3Q15, (P, P, P), **desc)
+    small.set_fields(geometry=np.zeros(P ** 3))
+    small.init_equilibrium(np.ones(P ** 3), [v.reshape(-1) for v in u_small])
+    small.run(0, steps, steps)
+    rs, us = small.get_macros()
+    es = small.kinetic_energy()
+    small.close()
+
+    print("[max size] tiling the field to 1024^3 (34 GB of host arrays)", flush=True)
+    n = L ** 3
+    reps = (L // P, L // P, L // P)
+    u_big = [np.tile(v, reps).reshape(-1) for v in u_small]
+    big = lbmc.Lattice(lbmc.D3Q15, (L, L, L), **desc)
+    print("[max size] 1024^3 lattice created; uploading", flush=True)
+    big.set_fields(geometry=np.zeros(n))
+    big.init_equilibrium(np.ones(n), u_big)
+    del u_big
+    print(f"[max size] {steps} calls", flush=True)
+    big.run(0, steps, steps)
+    eb = big.kinetic_energy()
+    rng = np.random.default_rng(11)
+    idx = np.concatenate([rng.integers(0, n, 2000), [0, n - 1, L - 1, L * L - 1, n - L * L]])
+    for i in idx:
+        x, y, z = int(i % L), int((i // L) % L), int(i // (L * L))
+        j = (x % P) + P * ((y % P) + P * (z % P))
+        rb, ub = big.get_node(int(i))
+        assert rb == rs[j] and np.array_equal(ub, us[:, j]), (int(i), j)
+    big.close()
+    print(f"[max size] E 1024^3 {eb:.12e}, 4096 x E 64^3 {4096 * es:.12e}", flush=True)
+    assert abs(eb - 4096 * es) <= 1e-9 * abs(eb)
