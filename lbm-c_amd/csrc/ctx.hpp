@@ -113,6 +113,7 @@ inline void fill_args(const lbmc_ctx *c, StepArgs<C> &a, long long t, int store)
     a.my = c->dy.m;
     a.ly = c->dy.l;
     a.remap = c->remap;
+    a.wide_rows = (c->key.wide && c->L[0] % 64 == 0) ? 1 : 0;
     a.tau = (C)c->desc.tau;
     a.c_smag = (C)c->desc.c_smag;
     for (int i = 0; i < 15; i++) a.tau_mrt[i] = (C)c->desc.tau_mrt[i];

@@ -297,6 +297,33 @@ __device__ __forceinline__ void node_update(const StepArgs<typename St::C> &a, i
     }
 }
 
+// n / d for 0 <= n < 2^31 with the magic pair (m, l) of d
+__device__ __forceinline__ int fast_div(int n, unsigned m, int l)
+{
+    return (int)((__umulhi((unsigned)n, m) + (unsigned)n) >> l);
+}
+
+// 64-bit-offset kernels on lattices with Lx % 64 == 0: the 64 nodes of a
+// wave lie in one x-row, so for direction i the start of the slot row,
+// (n - x) + dd[1][e_y+1] + dd[2][e_z+1], is the same for the whole wave and
+// goes into scalar registers; each lane keeps only its position in the row,
+// x + dd[0][e_x+1] < Lx, a 32-bit byte offset.  This is the uniform-base
+// plus 32-bit-offset access of the narrow kernels: the per-lane 64-bit
+// address form costs 18 % of the step on a 1024^3 FP32 lattice.
+template <class L, class St, class F>
+__device__ __forceinline__ void wide_row_slots(const StepArgs<typename St::C> &a, int n, const int (&dd)[3][3], F &&f)
+{
+    using S = typename St::S;
+    const int x = n - fast_div(n, a.mx, a.lx) * a.L[0];
+    const int r0 = n - x;
+    sfor<0, L::Q>([&](auto ii) {
+        constexpr int i = decltype(ii)::value;
+        const int rs = __builtin_amdgcn_readfirstlane(r0 + dd[1][L::ey[i] + 1] + dd[2][L::ez[i] + 1]);
+        char *base = (char *)((S *)a.f + (size_t)i * a.npad + (size_t)rs);
+        f(ii, (S *)(base + (unsigned)(x + dd[0][L::ex[i] + 1]) * (unsigned)sizeof(S)));
+    });
+}
+
 // index-shift gather of node n's populations (solver.cu:77-94) from the
 // slots n + dd[0][e_x+1] + dd[1][e_y+1] + dd[2][e_z+1]
 template <class L, class St, bool WIDE>
@@ -304,6 +331,12 @@ __device__ __forceinline__ void gather_node(const StepArgs<typename St::C> &a, i
                                             typename St::C (&v)[L::Q])
 {
     using S = typename St::S;
+    if constexpr (WIDE) {
+        if (a.wide_rows) {
+            wide_row_slots<L, St>(a, n, dd, [&](auto ii, const S *p) { v[decltype(ii)::value] = St::load(ldg(p)); });
+            return;
+        }
+    }
     const S *const fb = (const S *)a.f;
     sfor<0, L::Q>([&](auto ii) {
         constexpr int i = decltype(ii)::value;
@@ -332,6 +365,20 @@ __device__ __forceinline__ void scatter_node(const StepArgs<typename St::C> &a, 
         // member draw the same words for the same call
         const unsigned ts = (unsigned)a.s[0] + (unsigned)a.L[0] * ((unsigned)a.s[1] + (unsigned)a.L[1] * (unsigned)a.s[2]);
         r = sr_seed(n, ts);
+    }
+    if constexpr (WIDE) {
+        if (a.wide_rows) {
+            wide_row_slots<L, St>(a, n, dd, [&](auto ii, S *p) {
+                constexpr int i = decltype(ii)::value;
+                if constexpr (St::stochastic) {
+                    stg(p, St::store_sr(v[i], r));
+                    r = sr_next(r);
+                } else {
+                    stg(p, St::store(v[i]));
+                }
+            });
+            return;
+        }
     }
     sfor<0, L::Q>([&](auto ii) {
         constexpr int i = decltype(ii)::value;
@@ -367,11 +414,6 @@ __device__ __forceinline__ void step_node(const StepArgs<typename St::C> &a, int
     }
     node_update<L, St, FAM, GUO, SMAG, MFIX>(a, n, v);
     scatter_node<L, St, WIDE>(a, n, dd, v);
-}
-
-__device__ __forceinline__ int fast_div(int n, unsigned m, int l)
-{
-    return (int)((__umulhi((unsigned)n, m) + (unsigned)n) >> l);
 }
 
 // Slot offsets of node n for this call: the slot of direction i is

@@ -118,8 +118,9 @@ def _initial_state(lattice, L, seed):
     n = int(np.prod(L))
     rng = np.random.default_rng(seed)
     if dim == 3:
+        # a cube of side Lx, cut to (Lz, Ly, Lx) for box lattices
         ux, uy, uz = workloads.decaying_turbulence(L[0], k0=2.0, u_rms=0.04, seed=seed)
-        u = np.stack([ux.reshape(-1), uy.reshape(-1), uz.reshape(-1)])
+        u = np.stack([v[: L[2], : L[1], : L[0]].reshape(-1) for v in (ux, uy, uz)])
     else:
         u = 0.03 * rng.standard_normal((2, n))
     rho = 1.0 + 0.002 * rng.standard_normal(n)
@@ -390,3 +391,25 @@ def test_closure_c4_coarse_full_size():
     assert fb == 0.0 and edge > 1e-3
     assert np.max(np.abs(ud - uo)) < 1e-5
     assert np.max(np.abs(rd - ro)) < 1e-5
+
+
+@pytest.mark.parametrize("every", [1, 3])
+def test_closure_wide_offset_kernels_match(every, monkeypatch):
+    """The closure kernels' 64-bit-offset forms (forced on a small lattice;
+    Lx = 64, so they address slot rows from wave-uniform bases) give the
+    same bits as the 32-bit-offset forms, network evaluated and reused."""
+    L, steps = (64, 8, 6), 7
+    print(f"[closure] wide offsets every={every}", flush=True)
+    out = []
+    for wide in (False, True):
+        if wide:
+            monkeypatch.setenv("LBMC_FORCE_WIDE", "1")
+        band = (0.511, 0.57)
+        lat, m, _ = _pair(lbmc.D3Q15, L, 0.51, 0.17, lbmc.FP32, band, (0.004, 0.54), every=every)
+        lat.run(0, steps, 1)
+        out.append((lat.get_f(), lat.get_macros()))
+        lat.close()
+        m.close()
+    (fa, (ra, ua)), (fb, (rb, ub)) = out
+    assert np.array_equal(fa, fb)
+    assert np.array_equal(ra, rb) and np.array_equal(ua, ub)

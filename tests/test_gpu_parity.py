@@ -70,12 +70,15 @@ def test_fp32_tolerance(case):
     assert du <= K
 
 
-@pytest.mark.parametrize("case", [cases.turbulence_d3q15(L=10, steps=12), cases.pressure_d2q9(steps=50)],
-                         ids=["turb", "pressure"])
+@pytest.mark.parametrize("case", [cases.turbulence_d3q15(L=10, steps=12), cases.pressure_d2q9(steps=50),
+                                  cases.faces_d3q15(L=(64, 6, 5), steps=12),
+                                  cases.pressure_d2q9(Lx=64, Ly=24, steps=50)],
+                         ids=["turb", "pressure", "faces_lx64", "pressure_lx64"])
 def test_wide_offset_kernels(case, monkeypatch):
     """The 64-bit-offset kernels (used when a direction array spans 4 GiB or
     more) forced on a small lattice: FP64 bit-exact, FP32 equal to the
-    32-bit-offset kernels bit for bit."""
+    32-bit-offset kernels bit for bit.  With Lx % 64 == 0 (the lx64 cases)
+    they address each direction's slot row from a wave-uniform base."""
     o, g32 = run_pair(case, lbmc.FP32)
     monkeypatch.setenv("LBMC_FORCE_WIDE", "1")
     g64 = cases.build(lbmc.Lattice, case, storage=lbmc.FP64)
