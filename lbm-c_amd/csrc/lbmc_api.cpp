@@ -303,6 +303,15 @@ lbmc_status lbmc_create(const lbmc_desc *desc, lbmc_ctx **out)
     c->key.smag = (c->key.family == 0 && desc->c_smag > 0) ? 1 : 0;
     c->key.mfix = (desc->lattice == LBMC_D3Q15 && (desc->quirks & LBMC_FIX_D3Q15_MINV)) ? 1 : 0;
     c->key.wide = (unsigned long long)c->npad * c->sbytes >= (1ULL << 32) ? 1 : 0;
+    // Large D3Q15 FP32 lattices with Lx % 64 == 0 take the 64-bit-offset
+    // kernels below 4 GiB too: with Lx % 64 == 0 those address each wave's
+    // slot row from a wave-uniform base, measured faster than the narrow
+    // kernels' per-lane offsets from the array base once a direction array
+    // reaches ~1 GB (640^3 +15 %, 768^3 +12 %, 1024^2 x 1023 +27 %), equal at
+    // 384^3-512^3 and 3 % slower at 256^3.  Same results either way.
+    if (desc->lattice == LBMC_D3Q15 && desc->storage == LBMC_FP32 && desc->length[0] % 64 == 0 &&
+        (unsigned long long)c->npad * c->sbytes >= (768ULL << 20))
+        c->key.wide = 1;
     // test hook: run the 64-bit-offset kernels on a small lattice
     if (const char *env = getenv("LBMC_FORCE_WIDE"))
         if (atoi(env) != 0) c->key.wide = 1;
