@@ -176,3 +176,20 @@ def test_step_before_fields_is_state_error():
     with pytest.raises(lbmc.LbmcError) as e:
         lat.step(0)
     assert e.value.status == 4
+
+
+@pytest.mark.parametrize("cap", ["1", "2", "16"])
+def test_residency_cap_leaves_results_unchanged(cap, monkeypatch):
+    """The residency cap (LBMC_STEP_BLOCKS_PER_CU, dynamic LDS the step
+    kernels do not use) only changes how many blocks run at once: FP32 and
+    FP16S results are bit-identical to the uncapped run."""
+    case = cases.turbulence_d3q15(L=16, steps=9)
+    for storage in (lbmc.FP32, lbmc.FP16S):
+        out = []
+        for c in ("0", cap):
+            monkeypatch.setenv("LBMC_STEP_BLOCKS_PER_CU", c)
+            g = cases.build(lbmc.Lattice, case, storage=storage)
+            g.run(0, case["steps"], 3)
+            out.append(g.get_f())
+            g.close()
+        assert np.array_equal(out[0], out[1]), (cap, storage)
