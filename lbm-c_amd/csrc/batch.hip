@@ -107,7 +107,7 @@ lbmc_status upload_members(lbmc_batch *b, long long t0, int store_every)
     int any_eval = 0;  // the flag batch_advance_kernel keeps for later calls
     for (int k = 0; k < b->count; k++) {
         const lbmc_ctx *c = b->members[(size_t)k];
-        if (c->nsf || c->key.wide || !c->fields_set)
+        if (c->nsf || c->offsets64 || !c->fields_set)
             return bfail(b, LBMC_ERR_UNSUPPORTED, "batch member %d: sf nodes or state not supported", k);
         if (c->key.closure && (c->closure_model != b->closure_model || c->closure_lo != b->closure_lo ||
                                c->closure_hi != b->closure_hi))
@@ -285,7 +285,7 @@ lbmc_status lbmc_batch_create(lbmc_ctx *const *members, int count, lbmc_batch **
             closure_hi = c->closure_hi;
             closure_img = c->mlp_img;
         }
-        if (c->key.wide) return bfail(nullptr, LBMC_ERR_UNSUPPORTED, "lbmc_batch_create: member %d is too large", k);
+        if (c->offsets64) return bfail(nullptr, LBMC_ERR_UNSUPPORTED, "lbmc_batch_create: member %d is too large", k);
         for (int j = 0; j < k; j++)
             if (members[j] == c) return bfail(nullptr, LBMC_ERR_INVALID, "lbmc_batch_create: member %d repeats %d", k, j);
         nchunks += (c->N + kBlock - 1) / kBlock;

@@ -39,6 +39,8 @@ struct lbmc_ctx {
     bool fields_set = false;
     bool porous = false;
     bool all_fluid = false;  // every node plain fluid (mask all 0): the step skips the mask
+    bool offsets64 = false;  // a direction array spans 4 GiB or more: only 64-bit offsets reach it
+                             //   (key.wide may also be set below that, by choice)
     double F[3] = {0, 0, 0};
     int ncls = 1;
     uint16_t cls_host[256] = {};

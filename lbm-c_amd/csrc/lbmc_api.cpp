@@ -302,7 +302,8 @@ lbmc_status lbmc_create(const lbmc_desc *desc, lbmc_ctx **out)
     c->key.guo = desc->forcing ? 1 : 0;
     c->key.smag = (c->key.family == 0 && desc->c_smag > 0) ? 1 : 0;
     c->key.mfix = (desc->lattice == LBMC_D3Q15 && (desc->quirks & LBMC_FIX_D3Q15_MINV)) ? 1 : 0;
-    c->key.wide = (unsigned long long)c->npad * c->sbytes >= (1ULL << 32) ? 1 : 0;
+    c->offsets64 = (unsigned long long)c->npad * c->sbytes >= (1ULL << 32);
+    c->key.wide = c->offsets64 ? 1 : 0;
     // Large D3Q15 FP32 lattices with Lx % 64 == 0 take the 64-bit-offset
     // kernels below 4 GiB too: with Lx % 64 == 0 those address each wave's
     // slot row from a wave-uniform base, measured faster than the narrow

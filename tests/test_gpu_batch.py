@@ -247,3 +247,23 @@ def test_batch_with_closure_members_matches_single_lattices(every):
         assert np.max(np.abs(fg - fs)) <= 1e-6, spec["L"]
         assert np.max(np.abs(ug - us)) <= 1e-6, spec["L"]
     model.close()
+
+
+def test_batch_accepts_members_on_the_64bit_offset_kernels(monkeypatch):
+    """A member whose own steps take the 64-bit-offset kernels by choice
+    (forced here; the library picks them for large D3Q15 FP32 lattices)
+    while its arrays stay below 4 GiB joins a batch, which steps it with
+    32-bit offsets: the same bits as a batch of ordinary members, and as the
+    member stepped on its own."""
+    group = _members(lbmc.D3Q15)[:3]
+    print(f"[batch] forced-wide members={len(group)}", flush=True)
+    ref = _run_batch(group, lbmc.FP32, 0, 9, store_every=3)
+    monkeypatch.setenv("LBMC_FORCE_WIDE", "1")
+    got = _run_batch(group, lbmc.FP32, 0, 9, store_every=3)
+    for c, (fa, (ra, ua)), (fb, (rb, ub)) in zip(group, ref, got):
+        assert np.array_equal(fa, fb), c["name"]
+        assert np.array_equal(ra, rb) and np.array_equal(ua, ub), c["name"]
+        lat = cases.build(lbmc.Lattice, c, storage=lbmc.FP32)
+        lat.run(0, 9, 3)
+        assert np.array_equal(lat.get_f(), fa), c["name"]
+        lat.close()
