@@ -246,7 +246,8 @@ lbmc_status lbmc_set_closure_shift(lbmc_ctx *ctx, double dtau);
  * an addition.  Members are contexts made by lbmc_create and set up as usual
  * (fields and initial state); they share lattice type, storage and device,
  * may differ in size, collision type, forcing, tau and boundary codes, and
- * may not use sf boundaries.  Members with the learned closure share one
+ * may not use sf boundaries or direction arrays of 4 GiB or more
+ * (LBMC_ERR_UNSUPPORTED).  Members with the learned closure share one
  * model and band; each keeps its own evaluation interval, on the schedule
  * of lbmc_run.  Members must outlive the batch; their state is read at every
  * lbmc_batch_run, and lbmc_get_* on a member reads what the batch computed.
