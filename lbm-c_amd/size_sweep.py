@@ -1,4 +1,4 @@
-"""D3Q15 BGK + Smagorinsky step rate against lattice size.
+"""D3Q15 step rate against lattice size (BGK + Smagorinsky, or MRT).
 
 Each size is a periodic cube holding a 64^3 decaying-turbulence field
 (k0 4, u_rms 0.05, seed 7) tiled to fill it; after 60 untimed calls, three
@@ -8,6 +8,11 @@ The addressing form of the step follows the library's defaults
 which address slot rows from wave-uniform bases when Lx % 64 == 0).
 
     python lbm-c_amd/size_sweep.py --sizes 256 512 640 768 --storage fp32
+    python lbm-c_amd/size_sweep.py --sizes 256 --collision mrt
+
+MRT uses the corrected D3Q15 M^-1 (LBMC_FIX_D3Q15_MINV; the reference's
+table is all zeros, so its MRT relaxes nothing), tau 0.6: rates 1/tau on
+the stress moments and 1.2-1.6 on the others.
 
 One process per size keeps the device memory of one lattice at a time.
 """
@@ -21,7 +26,14 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 P = 64
 
 
-def one(L, storage):
+# D3Q15 MRT rates in the moment order of M (rho, e, eps, jx, qx, jy, qy, jz,
+# qz, 3pxx, pww, pxy, pyz, pxz, mxyz): stress moments at 1/tau
+def mrt_rates(tau):
+    s = 1.0 / tau
+    return [0.0, 1.6, 1.2, 0.0, 1.6, 0.0, 1.6, 0.0, 1.6, s, s, s, s, s, 1.2]
+
+
+def one(L, storage, collision):
     sys.path.insert(0, HERE)
     import numpy as np
 
@@ -34,7 +46,11 @@ def one(L, storage):
     n = L ** 3
     reps = (L // P,) * 3
     u = [np.tile(v, reps).reshape(-1) for v in u_small]
-    lat = lbmc.Lattice(lbmc.D3Q15, (L, L, L), lbmc.BGK, tau=0.51, c_smag=0.17, storage=st)
+    if collision == "mrt":
+        lat = lbmc.Lattice(lbmc.D3Q15, (L, L, L), lbmc.MRT, tau=0.6, tau_mrt=mrt_rates(0.6), storage=st,
+                           quirks=lbmc.FIX_D3Q15_MINV)
+    else:
+        lat = lbmc.Lattice(lbmc.D3Q15, (L, L, L), lbmc.BGK, tau=0.51, c_smag=0.17, storage=st)
     lat.set_fields(geometry=np.zeros(n))
     lat.init_equilibrium(np.ones(n), u)
     del u
@@ -45,7 +61,7 @@ def one(L, storage):
         t += 40
     lat.close()
     ms = min(windows)
-    return {"L": L, "storage": storage, "ms_per_step": [round(w, 4) for w in windows],
+    return {"L": L, "storage": storage, "collision": collision, "ms_per_step": [round(w, 4) for w in windows],
             "MLUPS": round(n / ms / 1e3, 1), "GBs": round(bpn * n / (ms * 1e-3) / 1e9, 1),
             "force_wide": os.environ.get("LBMC_FORCE_WIDE", "0")}
 
@@ -54,16 +70,17 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--sizes", type=int, nargs="+", default=[256, 512, 640, 768])
     ap.add_argument("--storage", default="fp32", choices=["fp64", "fp32", "fp16s", "bf16s"])
+    ap.add_argument("--collision", default="bgk", choices=["bgk", "mrt"])
     ap.add_argument("--one", type=int, default=0, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.one:
-        print(json.dumps(one(args.one, args.storage)), flush=True)
+        print(json.dumps(one(args.one, args.storage, args.collision)), flush=True)
         return
     for L in args.sizes:
         if L % P:
             sys.exit(f"size {L}: must be a multiple of {P}")
-        r = subprocess.run([sys.executable, "-u", os.path.abspath(__file__), "--one", str(L), "--storage", args.storage],
-                           check=False)
+        r = subprocess.run([sys.executable, "-u", os.path.abspath(__file__), "--one", str(L), "--storage", args.storage,
+                            "--collision", args.collision], check=False)
         if r.returncode != 0:
             sys.exit(r.returncode)
 
