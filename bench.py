@@ -79,7 +79,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-variants", action="store_true", help="skip the FP16S/BF16S lines")
+    ap.add_argument("--no-variants", action="store_true", help="skip the FP16S/BF16S/FP64 lines")
     ap.add_argument("--no-closure", action="store_true", help="skip the learned-closure step line")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble line")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -136,12 +136,14 @@ def main():
 
     variants = {}
     if not args.no_variants:
-        for st in (lbmc.FP16S, lbmc.BF16S):
+        # FP16S / BF16S: value-shifted 16-bit slots; FP64: the reference's own
+        # arithmetic, bit for bit
+        for st in (lbmc.FP16S, lbmc.BF16S, lbmc.FP64):
             v = make_lattice(lbmc, L, st, fields)
             vsteps = max(20, args.steps)
             v.time_steps(0, max(5, args.warmup), 0)
             vms = v.time_steps(args.warmup, vsteps, 0)
-            vb = 2 * 15 * 2
+            vb = 2 * 15 * (8 if st == lbmc.FP64 else 2)
             variants[lbmc.STORAGE_NAMES[st]] = {
                 "MLUPS": round(N * vsteps / (vms * 1e-3) / 1e6, 1),
                 "ms_per_step": round(vms / vsteps, 5),
