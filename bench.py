@@ -10,7 +10,9 @@ kernel over all 256^3 nodes, with the populations resident in HBM.
 
 Timing: W untimed warm-up steps, then exactly K steps between HIP events
 recorded on the library's stream (the stream the kernel runs on), with a
-stream synchronisation before and after.  One process, one GPU.
+stream synchronisation before and after.  One process, one GPU.  Five
+further windows of K/5 steps on the same lattice give the run-to-run
+spread ("repeats"); the reported value is the K-step window.
 
 Printed: one JSON line with MLUPS, the roofline of the step kernel
 (120 algorithmic bytes per node per step: 15 FP32 populations read and
@@ -117,7 +119,15 @@ def main():
     ms = lat.time_steps(args.warmup, args.steps, 0)
     lat.sync()
     wall = time.perf_counter() - wall0
-    lat.step(args.warmup + args.steps, True)
+    # five further windows on the same lattice, for the run-to-run spread
+    # (value is the K steps above)
+    t_next = args.warmup + args.steps
+    rwin = max(10, args.steps // 5)
+    repeats = []
+    for _ in range(5):
+        repeats.append(lat.time_steps(t_next, rwin, 0) / rwin)
+        t_next += rwin
+    lat.step(t_next, True)
     rho, u = lat.get_macros()
     finite = bool(np.all(np.isfinite(rho)) and np.all(np.isfinite(u)))
     lat.close()
@@ -254,6 +264,8 @@ def main():
                    "lattice": "D3Q15", "dims": [L, L, L], "tau": 0.51, "c_smag": 0.17,
                    "storage": "f32", "init": "f_eq(1, u), solenoidal k^4 exp(-2(k/4)^2), "
                    "u_rms 0.05, seed 2024", "parallelism": "single GPU"},
+        "repeats": {"steps_each": rwin, "ms_per_step": [round(r, 5) for r in repeats],
+                    "median_ms_per_step": round(sorted(repeats)[2], 5)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "bytes_per_node": bytes_node,
