@@ -60,18 +60,37 @@ def cpu_baseline(L, fields, seconds, threads):
     o.set_fields(geometry=np.zeros(L ** 3))
     o.init_equilibrium(np.ones(L ** 3), fields)
     o.step(0, False)  # warm-up
-    steps, t0 = 0, time.perf_counter()
-    while True:
-        o.step(1 + steps, False)
-        steps += 1
-        el = time.perf_counter() - t0
-        if steps >= 2 and el >= seconds:
-            break
+
+    def timed(t, min_seconds):
+        steps, t0 = 0, time.perf_counter()
+        while True:
+            o.step(t + steps, False)
+            steps += 1
+            el = time.perf_counter() - t0
+            if steps >= 2 and el >= min_seconds:
+                return steps, el
+
+    steps, el = timed(1, seconds)
+    # one thread, a short sample of the same lattice
+    pyoracle.set_threads(1)
+    steps1, el1 = timed(1 + steps, 1.0)
     o.close()
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": round(L ** 3 * steps / el / 1e6, 3), "unit": "MLUPS",
             "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle, same D3Q15 BGK+Smagorinsky 256^3 lattice and initial state, "
-                      f"{steps} steps in {el:.1f} s on {threads} OpenMP threads"}
+            "sample": f"FP64 oracle, same D3Q15 BGK+Smagorinsky {L}^3 lattice and initial state, "
+                      f"{steps} steps in {el:.1f} s on {threads} OpenMP threads",
+            "one_core_MLUPS": round(L ** 3 * steps1 / el1 / 1e6, 3),
+            "one_core_sample": f"{steps1} steps in {el1:.1f} s on 1 thread",
+            "nproc": os.cpu_count(), "cpu_model": model}
 
 
 def main():
