@@ -36,6 +36,13 @@
  *                       components (d3q15_zh_defs.cu:4-92 do not).
  *
  * Build with -ffp-contract=off: the point is the reference's exact rounding.
+ *
+ * Every function is pinned by closed forms, invariants or brute force in
+ * tests/test_oracle_*.py, and planted mistakes (ORC_MUTANT) show the pins
+ * bite.  Parity unpinned against the reference itself: the sf deviation
+ * above (only its intended semantics are pinned, by invariant) and the
+ * summation order of the kinetic energy (the reference sums with Thrust;
+ * the value agrees to rounding).
  */
 #include "lbmc_oracle.h"
 
@@ -540,11 +547,21 @@ typedef struct {
 
 static const zh_face zh_ref_faces[6] = {
     /* x: d3q15_zh_defs.cu:4-17 */
+#if ORC_MUTANT == 9
+    {0, 1, 0, {0, 3, 4, 5, 6}, {2, 14, 10, 12, 8}, 1, 2, 3, 4, 5, 6,
+     {{7, 14, 1, 1}, {11, 10, -1, -1}, {9, 12, -1, 1}, {13, 8, 1, -1}}},
+#else
     {0, 1, 0, {0, 3, 4, 5, 6}, {2, 14, 10, 12, 8}, 1, 2, 3, 4, 5, 6,
      {{7, 14, -1, 1}, {11, 10, -1, -1}, {9, 12, 1, 1}, {13, 8, 1, -1}}},
+#endif
     /* X: :19-32 */
+#if ORC_MUTANT == 11
+    {0, -1, 0, {0, 3, 4, 5, 6}, {1, 7, 11, 9, 13}, 2, 1, 3, 4, 5, 6,
+     {{14, 7, 1, -1}, {10, 11, 1, 1}, {12, 9, -1, -1}, {8, 13, -1, -1}}},
+#else
     {0, -1, 0, {0, 3, 4, 5, 6}, {1, 7, 11, 9, 13}, 2, 1, 3, 4, 5, 6,
      {{14, 7, 1, -1}, {10, 11, 1, 1}, {12, 9, -1, 1}, {8, 13, -1, -1}}},
+#endif
     /* y: :34-47 */
     {1, 1, 1, {0, 1, 2, 5, 6}, {4, 14, 10, 9, 13}, 3, 4, 1, 2, 5, 6,
      {{7, 14, -1, 1}, {11, 10, -1, -1}, {12, 9, 1, -1}, {8, 13, 1, 1}}},
@@ -554,8 +571,8 @@ static const zh_face zh_ref_faces[6] = {
     /* z: :64-77 */
     {2, 1, 2, {0, 1, 2, 3, 4}, {6, 14, 11, 12, 13}, 5, 6, 1, 2, 3, 4,
      {{7, 14, -1, 1}, {10, 11, 1, -1}, {9, 12, -1, -1}, {8, 13, 1, 1}}},
-    /* Z: :79-92 */
-    {2, -1, 0, {0, 1, 2, 3, 4}, {5, 7, 10, 9, 8}, 6, 5, 1, 2, 3, 4,
+    /* Z: :79-92 (its rho-u terms read u[0]; mutant 10 reads u[2]) */
+    {2, -1, ORC_MUTANT == 10 ? 2 : 0, {0, 1, 2, 3, 4}, {5, 7, 10, 9, 8}, 6, 5, 1, 2, 3, 4,
      {{14, 7, 1, -1}, {11, 10, -1, 1}, {12, 9, 1, 1}, {13, 8, -1, -1}}},
 };
 

@@ -183,24 +183,84 @@ def test_zou_he_d3q15_fixed_closure(face):
         assert set(changed) <= set(np.nonzero(e[axis] == side)[0])
 
 
+# Moments left by the reference's D3Q15 Zou-He faces (d3q15_zh_defs.cu:4-92),
+# derived by hand from its formulas.  Notation: A = f3 - f4, B = f5 - f6,
+# P = f1 - f2 (populations the face keeps); K = the sum of the four known
+# diagonals (e_n = -side), Kab = their sum weighted by e_a e_b (a < b the two
+# transverse axes); S = f0 + the four in-plane axis populations + 2 * (the
+# five incoming known populations); u_n = side * (1 - S / rho) is the normal
+# velocity the face stores.  Each face adds c = rho u_n / 12 to the four
+# unknown diagonals and subtracts a quarter of a signed sum of two of A, B, P;
+# the signs decide the transverse leftovers:
+#   face  sum f                         j = (jx, jy, jz)      Pi_ab          Pi_aa (a: first transverse axis)
+#   x     rho                           (rho u_x, 2A, 0)      2 Kab          f3+f4 + 2K + rho u_x / 3
+#   X     rho                           (rho u_x, 2A, B)      2 Kab + B      f3+f4 + 2K - rho u_x / 3
+#   y     rho                           (2P, rho u_y, 0)      2 Kab          f1+f2 + 2K + rho u_y / 3
+#   Y     rho                           (2P, rho u_y, 0)      2 Kab          f1+f2 + 2K - rho u_y / 3
+#   z     rho                           (2P, 0, rho u_z)      2 Kab          f1+f2 + 2K + rho u_z / 3
+#   Z     f0+f1+f2+f3+f4 + 2(f5+f7+f8+f9+f10) = rho (1 + u_z)
+#                                       (2P, 0, 0)            2 Kab          f1+f2 + 2K
+# Face Z computes its rho-u terms from u[0], which it has just set to 0
+# (:87-91): the unknowns copy their partners plus the transverse term only,
+# so neither mass nor normal momentum closes.
+ZH_REF = {
+    # face: (jx, jy, jz as functions of (f, rho u_n)), extra Pi_ab, sign of the rho u_n / 3 term
+    0: (lambda f, m: (m, 2 * (f[3] - f[4]), 0.0), lambda f: 0.0, 1),
+    1: (lambda f, m: (m, 2 * (f[3] - f[4]), f[5] - f[6]), lambda f: f[5] - f[6], -1),
+    2: (lambda f, m: (2 * (f[1] - f[2]), m, 0.0), lambda f: 0.0, 1),
+    3: (lambda f, m: (2 * (f[1] - f[2]), m, 0.0), lambda f: 0.0, -1),
+    4: (lambda f, m: (2 * (f[1] - f[2]), 0.0, m), lambda f: 0.0, 1),
+    5: (lambda f, m: (2 * (f[1] - f[2]), 0.0, 0.0), lambda f: 0.0, 0),
+}
+
+
+def test_zou_he_d3q15_moment_set_fixes_the_unknowns():
+    """The five functionals the reference-face pin checks (sum f, j_a, j_b,
+    Pi_ab, Pi_aa) are independent on each face's five unknown populations,
+    so the pin leaves no combination of the unknowns unchecked."""
+    e, *_ = tables(po.D3Q15)
+    for face in range(6):
+        axis, side = face // 2, (1 if face % 2 == 0 else -1)
+        a, b = [d for d in range(3) if d != axis]
+        unk = np.nonzero(e[axis] == side)[0]
+        rows = np.array([np.ones(15), e[a], e[b], e[a] * e[b], e[a] ** 2])[:, unk]
+        assert len(unk) == 5 and np.linalg.matrix_rank(rows) == 5
+
+
 @pytest.mark.parametrize("face", range(6))
 def test_zou_he_d3q15_reference_defects(face):
-    """The reference D3Q15 faces (d3q15_zh_defs.cu:4-92) close mass and normal
-    momentum on faces x..z, but leave transverse momentum; face Z computes its
-    u-terms from u[0] (= 0) and so does not close mass."""
-    e, w, *_ = tables(po.D3Q15)
+    """The reference D3Q15 faces (d3q15_zh_defs.cu:4-92) against the moments
+    derived by hand from them (table above): mass, all three momentum
+    components, the transverse shear moment and one normal-stress moment,
+    on random nodes, to FP64 rounding.  Only the unknown populations
+    change, rho is kept, the transverse velocities are zero."""
+    e, *_ = tables(po.D3Q15)
     o = po.Oracle(po.D3Q15, (4, 4, 4), micro_bc=True)
-    axis = face // 2
-    rho = 1.003
-    f = feq_numpy(po.D3Q15, 1.0, np.array([0.01, -0.02, 0.015])) + 1e-4 * RNG.standard_normal(15)
-    fo, r2, u2 = o.micro(face + 1, f, rho, np.zeros(3))
-    trans = [d for d in range(3) if d != axis]
-    if face < 5:
-        assert abs(fo.sum() - rho) < 1e-15
-        assert abs((e @ fo)[axis] - rho * u2[axis]) < 1e-16
-    else:
-        assert abs(fo.sum() - rho) > 1e-6
-    assert max(abs((e @ fo)[d]) for d in trans) > 1e-6
+    axis, side = face // 2, (1 if face % 2 == 0 else -1)
+    a, b = [d for d in range(3) if d != axis]
+    jfun, pab_extra, usign = ZH_REF[face]
+    known_diag = np.nonzero((e[axis] == -side) & (np.abs(e).sum(axis=0) == 3))[0]
+    inplane = np.nonzero((e[axis] == 0) & (np.abs(e).sum(axis=0) == 1))[0]
+    incoming = np.nonzero(e[axis] == -side)[0]
+    for _ in range(20):
+        rho = 1 + 0.01 * RNG.standard_normal()
+        f = feq_numpy(po.D3Q15, 1.0, 0.03 * RNG.standard_normal(3)) + 3e-3 * RNG.standard_normal(15)
+        fo, r2, u2 = o.micro(face + 1, f, rho, np.zeros(3))
+        S = f[0] + f[inplane].sum() + 2 * f[incoming].sum()
+        un = side * (1 - S / rho)
+        assert r2 == rho
+        assert abs(u2[axis] - un) < 1e-14 and u2[a] == 0.0 and u2[b] == 0.0
+        m = rho * u2[axis]
+        tol = 1e-15
+        assert abs(fo.sum() - (rho if face < 5 else S)) < tol
+        assert np.allclose(e @ fo, jfun(f, m), rtol=0, atol=tol)
+        K = f[known_diag].sum()
+        Kab = (e[a] * e[b] * f)[known_diag].sum()
+        assert abs((e[a] * e[b] * fo).sum() - (2 * Kab + pab_extra(f))) < tol
+        Paa = f[inplane][np.abs(e[a][inplane]) == 1].sum() + 2 * K + usign * m / 3
+        assert abs((e[a] ** 2 * fo).sum() - Paa) < tol
+        changed = np.nonzero(fo != f)[0]
+        assert set(changed) <= set(np.nonzero(e[axis] == side)[0])
 
 
 @pytest.mark.parametrize("lat,q,dim,L", LATTICES)

@@ -21,6 +21,12 @@ MUTANTS = {
     6: ("Guo velocity shift F/rho instead of F/2rho", "test_oracle_operators.py::test_guo_source_moments"),
     7: ("D2Q9 moment equilibrium sign of rho", "test_oracle_operators.py::test_mrt_conserves_and_reduces_to_bgk"),
     8: ("index shift x + t e instead of x - t e", "test_oracle_lattice.py::test_streaming_moves_population_by_e"),
+    9: ("D3Q15 Zou-He face x: transverse signs of two diagonals swapped",
+        "test_oracle_operators.py::test_zou_he_d3q15_reference_defects"),
+    10: ("D3Q15 Zou-He face Z: rho-u terms from u[2] instead of the reference's u[0]",
+         "test_oracle_operators.py::test_zou_he_d3q15_reference_defects"),
+    11: ("D3Q15 Zou-He face X: one diagonal's second transverse sign flipped",
+         "test_oracle_operators.py::test_zou_he_d3q15_reference_defects"),
 }
 
 
