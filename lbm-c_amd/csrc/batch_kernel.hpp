@@ -78,10 +78,10 @@ __global__ __launch_bounds__(kBlock) void batch_step_kernel(const BatchMember<ty
         const int variant = m.variant;
         if (evaluates(variant, a.tmode)) continue;  // batch_eval_kernel's
         for (int j = 0; j < sub; j++) {
-            const int t0 = ch.y + j * kBlock;  // first node of the tile
-            if (t0 >= a.nnodes) break;         // block-uniform
-            const int n0 = t0 + (int)threadIdx.x;
-            if (n0 >= a.nnodes) continue;
+            const unsigned t0 = tile_node((unsigned)ch.y, (unsigned)j, kBlock, 0u);  // first node of the tile
+            if (t0 >= (unsigned)a.nnodes) break;                                  // block-uniform
+            if (t0 + threadIdx.x >= (unsigned)a.nnodes) continue;
+            const int n0 = (int)(t0 + threadIdx.x);
             int dd[3][3];
             slot_offsets<L>(a, n0, dd);
             if constexpr (CL) {
@@ -128,11 +128,11 @@ __global__ __launch_bounds__(kBlock, 4) void batch_eval_kernel(const BatchMember
         const int variant = m.variant;
         if (!evaluates(variant, a.tmode)) continue;  // block-uniform
         for (int j = 0; j < sub; j++) {
-            const int t0 = ch.y + j * kBlock;
-            if (t0 >= a.nnodes) break;
-            const int n0 = t0 + (int)threadIdx.x;
-            const bool live = n0 < a.nnodes;
-            const int n = live ? n0 : a.nnodes - 1;
+            const unsigned t0 = tile_node((unsigned)ch.y, (unsigned)j, kBlock, 0u);
+            if (t0 >= (unsigned)a.nnodes) break;
+            const unsigned n0 = t0 + threadIdx.x;
+            const bool live = n0 < (unsigned)a.nnodes;
+            const int n = live ? (int)n0 : a.nnodes - 1;
             int dd[3][3];
             slot_offsets<L>(a, n, dd);
             float v[L::Q];

@@ -16,6 +16,7 @@
 #pragma once
 
 #include "mlp.hpp"
+#include "node_index.hpp"
 #include "step_kernel.hpp"
 
 namespace lbmc {
@@ -101,11 +102,11 @@ __global__ __launch_bounds__(kBlock, 4) void closure_step_kernel(const StepArgs<
     }
     __syncthreads();
     mlp_v8b *xw = xb + (threadIdx.x & ~63u);
-    const int nchunks = (a.nnodes + kBlock - 1) / kBlock;
-    for (int b = (int)blockIdx.x; b < nchunks; b += (int)gridDim.x) {
-        const int n0 = b * kBlock + (int)threadIdx.x;
-        const bool live = n0 < a.nnodes;
-        const int n = live ? n0 : a.nnodes - 1;
+    const unsigned nchunks = tile_count(a.nnodes, kBlock);
+    for (unsigned b = blockIdx.x; b < nchunks; b += gridDim.x) {
+        const unsigned n0 = tile_node(0u, b, kBlock, threadIdx.x);
+        const bool live = n0 < (unsigned)a.nnodes;
+        const int n = live ? (int)n0 : a.nnodes - 1;
         int dd[3][3];
         slot_offsets<L>(a, n, dd);
         float v[L::Q];

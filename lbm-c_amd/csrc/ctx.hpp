@@ -59,7 +59,7 @@ struct lbmc_ctx {
     bool tcache_valid = false;
 
     // launch shape
-    int grid = 1, remap = 0;
+    int grid = 1;
     int closure_grid = 1;  // blocks of the grid-stride closure kernel
     int reuse_grid = 1;    // blocks of the closure-reuse kernel (one per 256 nodes)
     lbmc::FastDiv dx{}, dy{};
@@ -114,7 +114,6 @@ inline void fill_args(const lbmc_ctx *c, StepArgs<C> &a, long long t, int store)
     a.lx = c->dx.l;
     a.my = c->dy.m;
     a.ly = c->dy.l;
-    a.remap = c->remap;
     a.wide_rows = (c->key.wide && c->L[0] % 64 == 0) ? 1 : 0;
     a.tau = (C)c->desc.tau;
     a.c_smag = (C)c->desc.c_smag;

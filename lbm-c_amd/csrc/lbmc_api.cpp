@@ -150,17 +150,12 @@ int default_blocks_per_cu(const lbmc_ctx *c)
     }
 }
 
-// Launch shape: one thread per node, 256-thread blocks.  With a grid that is
-// a multiple of 8 the blocks can be reordered so that each XCD works on a
-// contiguous range of nodes (LBMC_XCD_REMAP=1 turns it on).
+// Launch shape: one thread per node, 256-thread blocks.
 void choose_shape(lbmc_ctx *c)
 {
     c->grid = (int)((c->N + kBlock - 1) / kBlock);
     c->dx = make_fastdiv(c->L[0]);
     c->dy = make_fastdiv(c->L[1]);
-    c->remap = 0;
-    if (const char *env = getenv("LBMC_XCD_REMAP"))
-        if (atoi(env) != 0 && c->grid % 8 == 0) c->remap = 1;
     // the closure kernel stages its weights once per block and loops over
     // 256-node chunks: a few blocks per CU
     int cap = 2048;
@@ -290,10 +285,6 @@ lbmc_status lbmc_create(const lbmc_desc *desc, lbmc_ctx **out)
     for (int d = 0; d < 3; d++) c->L[d] = d < D ? desc->length[d] : 1;
     c->N = N;
     c->npad = (N + 63) / 64 * 64;
-    // tuning hook: extra elements between direction arrays (staggers their
-    // base addresses across HBM channels)
-    if (const char *env = getenv("LBMC_PAD"))
-        if (atoi(env) > 0) c->npad += (atoi(env) + 63) / 64 * 64;
     c->storage = desc->storage;
     c->sbytes = desc->storage == LBMC_FP64 ? 8 : desc->storage == LBMC_FP32 ? 4 : 2;
     c->cbytes = desc->storage == LBMC_FP64 ? 8 : 4;

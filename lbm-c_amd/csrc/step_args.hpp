@@ -60,7 +60,6 @@ struct StepArgs {
     int nnodes;              // Lx * Ly * Lz
     unsigned mx, my;         // magic multipliers: n / Lx and row / Ly
     int lx, ly;              //   (q = (umulhi(n, m) + n) >> l, n < 2^31)
-    int remap;               // XCD-contiguous block order (grid % 8 == 0)
     int wide_rows;           // 64-bit-offset kernels with Lx % 64 == 0: every wave lies in
                              //   one x-row, so each direction's slot row is wave-uniform
     // slot offsets of this call relative to the node index: for e = +1 along

@@ -448,15 +448,8 @@ __device__ __forceinline__ void slot_offsets(const StepArgs<C> &a, int n, int (&
 template <class L, class St, int FAM, bool GUO, bool SMAG, bool MFIX, bool WIDE>
 __global__ __launch_bounds__(kBlock) void step_kernel(const StepArgs<typename St::C> a)
 {
-    int b = (int)blockIdx.x;
-    if (a.remap) {
-        // blocks are dispatched round-robin over the 8 XCDs; give each XCD a
-        // contiguous range of nodes so that the lines shared by neighbouring
-        // blocks sit in one L2
-        const int g8 = (int)gridDim.x >> 3;
-        b = (b & 7) * g8 + (b >> 3);
-    }
-    const int n = b * kBlock + (int)threadIdx.x;
+    // grid = ceil(nnodes / 256) <= 2^23 blocks, so n <= 2^31 - 1
+    const int n = (int)blockIdx.x * kBlock + (int)threadIdx.x;
     if (n >= a.nnodes) return;
     int dd[3][3];
     slot_offsets<L>(a, n, dd);

@@ -238,8 +238,11 @@ static int read_dims(hid_t zone, int *dim, int *cells)
     hid_t d = H5Dopen2(zone, " data", H5P_DEFAULT);
     if (d < 0) return -1;
     hid_t s = H5Dget_space(d);
-    hsize_t dims[2];
-    int nd = H5Sget_simple_extent_dims(s, dims, NULL);
+    hsize_t dims[2] = {0, 0};
+    /* the zone's size array is 3 x dim (reversed: dims = {3, dim}); check the
+     * rank before reading the extent into a two-entry array */
+    int nd = H5Sget_simple_extent_ndims(s);
+    if (nd == 2) nd = H5Sget_simple_extent_dims(s, dims, NULL);
     int isize[9];
     int rc = (nd != 2 || dims[0] != 3 || dims[1] < 2 || dims[1] > 3) ? -1 : 0;
     if (!rc) rc = H5Dread(d, H5T_NATIVE_INT32, H5S_ALL, H5S_ALL, H5P_DEFAULT, isize) < 0 ? -1 : 0;

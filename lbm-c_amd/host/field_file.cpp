@@ -14,6 +14,16 @@ namespace {
 
 const char kMagic[8] = {'L', 'B', 'M', 'C', 'F', 'L', 'D', '1'};
 
+// bytes between the read position and the end of the file
+uint64_t bytes_left(FILE *fp)
+{
+    const long pos = ftell(fp);
+    if (pos < 0 || fseek(fp, 0, SEEK_END) != 0) return 0;
+    const long end = ftell(fp);
+    if (end < pos || fseek(fp, pos, SEEK_SET) != 0) return 0;
+    return (uint64_t)(end - pos);
+}
+
 bool read_records(FILE *fp, std::vector<FieldRecord> *out, std::string *err)
 {
     char magic[8];
@@ -35,6 +45,13 @@ bool read_records(FILE *fp, std::vector<FieldRecord> *out, std::string *err)
             return false;
         }
         rec.name = name;
+        // the count comes from the file: check it against what the file
+        // still holds before allocating for it
+        const uint64_t esize = rec.type == 1 ? 8 : 4;
+        if ((rec.type == 1 || rec.type == 2) && count > bytes_left(fp) / esize) {
+            if (err) *err = "truncated record " + rec.name;
+            return false;
+        }
         if (rec.type == 1) {
             rec.f64.resize(count);
             if (fread(rec.f64.data(), 8, count, fp) != count) {
