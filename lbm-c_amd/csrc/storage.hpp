@@ -11,7 +11,8 @@
 // slot drops from ~3e-8 to ~1e-10); density is accumulated as 1 + sum g_i and
 // momentum as sum e_i g_i (the rest weights sum to one and carry no momentum).
 // FP16S scales g by 2^12 so that deviations down to ~1e-8 stay in the
-// normal range of _Float16 (|g| up to 16 fits).
+// normal range of _Float16; |g| < 65504 / 4096 (about 15.99) fits, larger
+// deviations store +-inf.
 //
 // The 16-bit formats round the step's stores stochastically (store_sr, the
 // hardware's stochastic-rounding conversion, which draws on the high bits of
