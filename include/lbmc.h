@@ -150,6 +150,18 @@ lbmc_status lbmc_get_node(lbmc_ctx *ctx, long long idx, double *rho, double *u);
 lbmc_status lbmc_get_f(lbmc_ctx *ctx, double *const *f);
 lbmc_status lbmc_set_f(lbmc_ctx *ctx, const double *const *f);
 
+/* lbmc_get_f without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * between graph captures, tight serving or checkpoint loops, harnesses that
+ * forbid hipMalloc).  lbmc_get_f allocates an N-double conversion buffer on
+ * every call; this variant converts through a fixed 8 MiB device array
+ * (1,048,576 doubles, one per GPU for the life of the process), one chunk of
+ * at most that many slots of one direction at a time, so lattices of any
+ * size are read with the same results, bit for bit.  Calls that overlap on
+ * one GPU (other threads, other contexts on the same device) share the array
+ * and race: serialise them. */
+lbmc_status lbmc_get_f_noalloc(lbmc_ctx *ctx, double *const *f);
+
 /* E = sum_n 1/2 rho |u|^2 over the stored macro arrays (the energy functor
  * and thrust::reduce of LBM-C/index.cu:315-370), accumulated in FP64 in a
  * fixed order. */

@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -698,6 +699,62 @@ lbmc_status lbmc_get_f(lbmc_ctx *c, double *const *f)
     }
     (void)hipFree(tmp);
     if (e != hipSuccess) return hip_fail(c, e, "get_f");
+    return LBMC_OK;
+}
+
+// Device scratch of lbmc_get_f_noalloc: one chunk of one direction's slots,
+// converted to doubles.  Each GPU the library runs on holds its own copy for
+// the life of the process.  Host code never passes the array's name to a
+// kernel or a copy (on the host it is only a stand-in for the symbol); it uses
+// the device address that hipGetSymbolAddress returns for the current device.
+constexpr long long kNoallocChunk = 1LL << 20;  // doubles: 8 MiB
+static __device__ __align__(256) double scratch[kNoallocChunk];
+
+namespace {
+constexpr int kNoallocMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kNoallocMaxDevices] = {};
+
+// Device address of `scratch` on the current device (set up once per device).
+lbmc_status scratch_on_device(lbmc_ctx *c, double **out)
+{
+    const int dev = c->desc.device;
+    if (dev < 0 || dev >= kNoallocMaxDevices)
+        return fail(c, LBMC_ERR_UNSUPPORTED, "lbmc_get_f_noalloc: device %d beyond the %d the scratch table holds",
+                    dev, kNoallocMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIP_TRY(c, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "lbmc_get_f_noalloc: scratch address");
+        if (!p) return fail(c, LBMC_ERR_DEVICE, "lbmc_get_f_noalloc: scratch address is null");
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return LBMC_OK;
+}
+}  // namespace
+
+lbmc_status lbmc_get_f_noalloc(lbmc_ctx *c, double *const *f)
+{
+    if (!c || !f) return fail(c, LBMC_ERR_INVALID, "null argument");
+    HIP_TRY(c, hipSetDevice(c->desc.device), "hipSetDevice");
+    double *buf = nullptr;
+    if (lbmc_status st = scratch_on_device(c, &buf)) return st;
+    const long long N = c->N;
+    for (int i = 0; i < c->Q; i++) {
+        if (!f[i]) continue;
+        for (long long k0 = 0; k0 < N; k0 += kNoallocChunk) {
+            const long long cnt = std::min(kNoallocChunk, N - k0);
+            // slots k0 .. k0 + cnt - 1 of direction i: the same kernel as
+            // lbmc_get_f, on the population base advanced by k0 slots
+            const void *base = (const char *)c->f + (size_t)k0 * (size_t)c->sbytes;
+            HIP_TRY(c, launch_f_to_double(c->desc.lattice, c->storage, base, c->npad, i, cnt, buf, c->stream),
+                    "get_f_noalloc launch");
+            HIP_TRY(c, hipStreamSynchronize(c->stream), "get_f_noalloc sync");
+            HIP_TRY(c, hipMemcpy(f[i] + k0, buf, sizeof(double) * (size_t)cnt, hipMemcpyDeviceToHost),
+                    "get_f_noalloc read");
+        }
+    }
     return LBMC_OK;
 }
 

@@ -35,6 +35,7 @@ EXPORTS = [
     "lbmc_create", "lbmc_destroy", "lbmc_last_error", "lbmc_set_fields", "lbmc_get_node_types",
     "lbmc_get_links", "lbmc_init_static", "lbmc_init_equilibrium", "lbmc_step", "lbmc_run",
     "lbmc_time_steps", "lbmc_get_macros", "lbmc_get_node", "lbmc_get_f", "lbmc_set_f",
+    "lbmc_get_f_noalloc",
     "lbmc_kinetic_energy", "lbmc_sync", "lbmc_info", "lbmc_copy_bandwidth",
     "lbmc_mlp_create", "lbmc_mlp_destroy", "lbmc_mlp_last_error", "lbmc_mlp_set_params",
     "lbmc_mlp_get_params", "lbmc_mlp_set_scaling", "lbmc_mlp_forward", "lbmc_mlp_loss_grad",
@@ -103,6 +104,7 @@ def _load():
                                      ct.POINTER(ct.c_double)]),
         "lbmc_get_f": (ct.c_int, [P, dpp]),
         "lbmc_set_f": (ct.c_int, [P, dpp]),
+        "lbmc_get_f_noalloc": (ct.c_int, [P, dpp]),
         "lbmc_kinetic_energy": (ct.c_int, [P, ct.POINTER(ct.c_double)]),
         "lbmc_sync": (ct.c_int, [P]),
         "lbmc_info": (ct.c_int, [P, ct.POINTER(ct.c_longlong), ct.POINTER(ct.c_int),
@@ -296,6 +298,12 @@ class Lattice:
     def get_f(self):
         f = [np.empty(self.n) for _ in range(self.q)]
         self._check(_lib.lbmc_get_f(self._h, _ptr_array(f, 15)))
+        return np.stack(f)
+
+    def get_f_noalloc(self):
+        """get_f through the library's fixed device scratch (no allocation)."""
+        f = [np.empty(self.n) for _ in range(self.q)]
+        self._check(_lib.lbmc_get_f_noalloc(self._h, _ptr_array(f, 15)))
         return np.stack(f)
 
     def set_f(self, f):
