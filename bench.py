@@ -8,11 +8,14 @@ E(k) ~ k^4 exp(-2 (k/k0)^2), k0 = 4, per-component u_rms = 0.05, f = f_eq
 (rho = 1, u) at t = 0; FP32 populations.  A step is one call of the fused
 kernel over all 256^3 nodes, with the populations resident in HBM.
 
-Timing: W untimed warm-up steps, then exactly K steps between HIP events
+Timing: after set-up, ~50 ms of a plain device copy (the roofline's
+bandwidth reference; it also lifts the GPU from the clocks an idle gap
+leaves), W untimed warm-up steps, then exactly K steps between HIP events
 recorded on the library's stream (the stream the kernel runs on), with a
 stream synchronisation before and after.  One process, one GPU.  Five
-further windows of K/5 steps on the same lattice give the run-to-run
-spread ("repeats"); the reported value is the K-step window.
+further windows of K/5 steps (at least 10) on the same lattice give the
+run-to-run spread ("repeats"); the reported value is the K-step window,
+"median_value" the MLUPS of the median repeat window.
 
 Printed: one JSON line with MLUPS, the roofline of the step kernel
 (120 algorithmic bytes per node per step: 15 FP32 populations read and
@@ -104,6 +107,10 @@ def main():
     ap.add_argument("--no-closure", action="store_true", help="skip the learned-closure step line")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble line")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--trace-calls", type=int, default=0, metavar="N",
+                    help="diagnostic instead of the benchmark: per-call step times (HIP events) of "
+                         "N calls of the workload, started after a 2 s idle gap and after the "
+                         "copy preload, FP32 and FP16S; prints one JSON line")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -123,15 +130,41 @@ def main():
     fields = workloads.decaying_turbulence(L, k0=4.0, u_rms=0.05, seed=2024)
     fields = tuple(v.reshape(-1) for v in fields)
 
-    # bandwidth reference: a plain 1 GiB copy (reads + writes 1 GiB each)
-    nbytes = 1 << 30
-    reps = 20
-    copy_ms = lbmc.copy_bandwidth(nbytes, reps)
-    copy_gbs = 2.0 * nbytes * reps / (copy_ms * 1e-3) / 1e9
-    log(f"bench: copy kernel {copy_gbs:.0f} GB/s")
+    def preload():
+        """A plain 1 GiB copy (reads + writes 1 GiB each) for ~50 ms right
+        before a warm-up: the bandwidth reference of the roofline, and it
+        brings the GPU out of the idle that host-side set-up leaves, so that
+        the timed steps run at the clocks of a long run.  After an idle gap
+        the power controller lowers the shader clock for ~10-25 ms of load
+        (to ~1.8 GHz from ~2.4 GHz in per-dispatch counter samples), whatever
+        the lattice holds; a 20-step window started from idle times that
+        ramp, not the step."""
+        nbytes, reps = 1 << 30, 150
+        ms_ = lbmc.copy_bandwidth(nbytes, reps)
+        return 2.0 * nbytes * reps / (ms_ * 1e-3) / 1e9
+
+    if args.trace_calls > 0:
+        # the same lattice state (a fresh start) timed call by call, once
+        # after the GPU idled for 2 s and once right after the preload
+        out = {"trace_calls": args.trace_calls, "unit": "us per call", "workload": f"{L}^3"}
+        for st in (lbmc.FP32, lbmc.FP16S):
+            for start in ("idle", "preload"):
+                tl = make_lattice(lbmc, L, st, fields)
+                if start == "idle":
+                    time.sleep(2.0)
+                else:
+                    preload()
+                per = [round(tl.time_steps(t, 1, 0) * 1e3, 1) for t in range(args.trace_calls)]
+                tl.close()
+                out[f"{lbmc.STORAGE_NAMES[st]}_{start}"] = per
+                log(f"bench: {lbmc.STORAGE_NAMES[st]} after {start}: " + " ".join(f"{v:.0f}" for v in per))
+        print(json.dumps(out), flush=True)
+        return
 
     log(f"bench: FP32 lattice {L}^3, warmup {args.warmup}, timed {args.steps}")
     lat = make_lattice(lbmc, L, lbmc.FP32, fields)
+    copy_gbs = preload()
+    log(f"bench: copy kernel {copy_gbs:.0f} GB/s")
     lat.time_steps(0, args.warmup, 0)
     lat.sync()
     wall0 = time.perf_counter()
@@ -170,6 +203,7 @@ def main():
         for st in (lbmc.FP16S, lbmc.BF16S, lbmc.FP64):
             v = make_lattice(lbmc, L, st, fields)
             vsteps = max(20, args.steps)
+            preload()
             v.time_steps(0, max(5, args.warmup), 0)
             vms = v.time_steps(args.warmup, vsteps, 0)
             vb = 2 * 15 * (8 if st == lbmc.FP64 else 2)
@@ -197,6 +231,7 @@ def main():
             c = make_lattice(lbmc, L, lbmc.FP32, fields)
             c.set_closure(model)
             c.set_closure_interval(every)
+            preload()
             c.time_steps(0, max(8, args.warmup // 5), 0)
             csteps = max(32, args.steps // 5)
             cms = c.time_steps(args.warmup, csteps, 0)
@@ -242,6 +277,7 @@ def main():
             grid, _ = b.autotune(0, steps_per_trial=16)
             _, chunk_nodes, bpc = b.shape()
             esteps = max(64, args.steps)
+            preload()
             b.time_steps(1000, 64, 0)
             ems = b.time_steps(1064, esteps, 0)
             b.close()
@@ -283,6 +319,7 @@ def main():
                    "lattice": "D3Q15", "dims": [L, L, L], "tau": 0.51, "c_smag": 0.17,
                    "storage": "f32", "init": "f_eq(1, u), solenoidal k^4 exp(-2(k/4)^2), "
                    "u_rms 0.05, seed 2024", "parallelism": "single GPU"},
+        "median_value": round(N / (sorted(repeats)[2] * 1e-3) / 1e6, 1),
         "repeats": {"steps_each": rwin, "ms_per_step": [round(r, 5) for r in repeats],
                     "median_ms_per_step": round(sorted(repeats)[2], 5)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

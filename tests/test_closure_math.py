@@ -1,6 +1,8 @@
 """CPU pins of the closure MLP oracle (oracle/pymlp.py): the GELU
-approximation against erf, the backward pass against central finite
-differences, and the bf16 rounding against its definition."""
+approximation against erf (exact-arithmetic form, and the device's fp16
+evaluation over every finite fp16 input), the backward pass against central
+finite differences, and the bf16 / fp16 roundings against their
+definitions."""
 import numpy as np
 from scipy.special import erf
 
@@ -14,6 +16,50 @@ def test_gelu_matches_normal_cdf():
     assert np.max(np.abs(pymlp.gelu(x) - x * phi_true)) < 2.9e-3
     # symmetric: Phi(-x) = 1 - Phi(x)
     assert np.allclose(pymlp.phi(-x), 1 - pymlp.phi(x), atol=1e-15)
+
+
+def _all_fp16():
+    h = np.arange(0x10000, dtype=np.uint32).astype(np.uint16).view(np.float16)
+    return h[np.isfinite(h)].astype(np.float64)
+
+
+def test_gelu16_matches_normal_cdf_on_every_fp16_input():
+    x = _all_fp16()
+    phi_true = 0.5 * (1 + erf(x / np.sqrt(2)))
+    s = pymlp.phi16(x)
+    assert not np.any(np.isnan(s))
+    assert np.max(np.abs(s - phi_true)) < 1.8e-3
+    # beyond the fit interval the unclamped polynomial saturates exactly
+    assert np.all(s[x > 3.5] == 1.0) and np.all(s[x < -3.5] == 0.0)
+    g = pymlp.gelu16(x)
+    small = np.abs(x) <= 8
+    assert np.max(np.abs(g[small] - x[small] * phi_true[small])) < 5.2e-3
+    assert np.array_equal(g[x > 3.5], x[x > 3.5]) and np.all(g[x < -3.5] == 0.0)
+    # Phi(-x) = 1 - Phi(x) up to one fp16 rounding of the last sum
+    assert np.max(np.abs(pymlp.phi16(-x) - (1 - s))) <= 2.0 ** -11
+
+
+def test_fp16_rounding():
+    f = pymlp._fp16
+    assert f(1.0 + 2 ** -11) == 1.0                  # tie -> even
+    assert f(1.0 + 3 * 2 ** -11) == 1.0 + 2 ** -9    # tie -> even (up)
+    assert f(1.0 + 2 ** -11 + 2 ** -30) == 1.0 + 2 ** -10
+    assert f(65519.0) == 65504.0 and np.isinf(f(65520.0))
+    assert f(2.0 ** -24) == 2.0 ** -24 and f(2.0 ** -26) == 0.0   # subnormals kept, then to zero
+    # the device's coefficients are the fp16 roundings of the fit's
+    assert np.all(np.abs(pymlp.GELU_C16 / pymlp.GELU_C - 1) < 2.5e-3)
+
+
+def test_dgelu16_close_to_exact_derivative():
+    # The fp16 Phi reaches its clamp values 0 / 1 a little inside +-3.5
+    # (from |x| ~ 2.9 on, where 1 - Phi < 2e-3), and there the derivative
+    # loses its z Phi' term (<= 0.01 for |x| < 3.3); beyond 3.7 both forms
+    # are exactly 0 or 1.
+    x = np.linspace(-6, 6, 24001)
+    d = np.abs(pymlp.dgelu16(x) - pymlp.dgelu(x))
+    assert np.max(d[np.abs(x) < 2.8]) < 1e-3
+    assert np.max(d[np.abs(x) < 3.3]) < 1.2e-2
+    assert np.all(d[np.abs(x) > 3.7] == 0.0)
 
 
 def test_dgelu_matches_finite_difference():

@@ -20,6 +20,18 @@ def test_gelu_pins_catch_wrong_coefficient(monkeypatch):
     _fails(pins.test_gelu_matches_normal_cdf)
 
 
+def test_gelu16_pin_catches_wrong_coefficient(monkeypatch):
+    c = pymlp.GELU_C16.copy()
+    c[2] = pymlp._fp16(1.01 * c[2])  # a 1 % slip in the quintic coefficient
+    monkeypatch.setattr(pymlp, "GELU_C16", c)
+    _fails(pins.test_gelu16_matches_normal_cdf_on_every_fp16_input)
+
+
+def test_gelu16_pin_catches_missing_output_clamp(monkeypatch):
+    monkeypatch.setattr(pymlp, "phi16", lambda z: pymlp._phi16_raw(pymlp._fp16(np.asarray(z, np.float64))))
+    _fails(pins.test_gelu16_matches_normal_cdf_on_every_fp16_input)
+
+
 def test_derivative_pin_catches_dropped_term(monkeypatch):
     monkeypatch.setattr(pymlp, "dgelu", lambda x: pymlp.phi(x))  # x Phi'(x) dropped
     _fails(pins.test_dgelu_matches_finite_difference)

@@ -39,10 +39,14 @@ def test_forward_matches_oracle():
     y0, _ = pymlp.forward(p, xn, quant=False)
     scale = np.max(np.abs(yq))
     err = np.abs(y - yq)
-    # FP32 accumulation vs FP64 with the same bf16 operands: a bf16 rounding of
-    # an activation may land on the other side, one ulp (2^-8) of that value
-    assert np.median(err) < 1e-5 * scale
-    assert np.max(err) < 3e-3 * scale
+    print(f"[closure] forward: median {np.median(err) / scale:.2e}, max {np.max(err) / scale:.2e} of max|y|",
+          flush=True)
+    # FP32 accumulation vs FP64 with the same fp16 activation arithmetic and
+    # bf16 operands: typical differences are FP32 summation error (~1e-8 of
+    # max|y|); an fp16 or bf16 rounding that lands on the other side of a tie
+    # moves one activation by one ulp, which bounds the worst sample
+    assert np.median(err) < 1e-6 * scale
+    assert np.max(err) < 1e-3 * scale
     # and the quantised network stays close to the exact one
     assert np.max(np.abs(y - y0)) < 0.05 * np.max(np.abs(y0))
     m.close()
@@ -58,15 +62,19 @@ def test_loss_and_gradient_match_oracle():
     m = lbmc.Mlp(pymlp.flatten(p))
     loss, g = m.loss_grad(x, t)
     lq, gq = pymlp.loss_and_grad(p, x.astype(np.float64), t.astype(np.float64), quant=True)
-    assert abs(loss - lq) < 1e-4 * lq
+    assert abs(loss - lq) < 1e-5 * lq
     gv = pymlp.flatten(gq)
     for name, sl in (("w1", slice(0, 512)), ("w2", slice(512, 4608)), ("b2", slice(4608, 4672)),
                      ("w3", slice(4672, 4736)), ("b3", slice(4736, 4737))):
         d = np.linalg.norm(g[sl] - gv[sl]) / np.linalg.norm(gv[sl])
-        assert d < 1e-2, (name, d)
+        print(f"[closure] gradient {name}: relative error {d:.2e}", flush=True)
+        # the oracle emulates every rounding point of the device (fp16
+        # activations, bf16 operands and error signals): what is left is FP32
+        # summation order, ~1e-6 of the gradient's norm
+        assert d < 1e-4, (name, d)
     # bias column of w1 is the sum of the first-layer error signal
     assert np.allclose(g[0:512].reshape(64, 8)[:, 7], gv[0:512].reshape(64, 8)[:, 7],
-                       atol=2e-2 * np.max(np.abs(gv[0:512])))
+                       atol=1e-4 * np.max(np.abs(gv[0:512])))
     m.close()
 
 

@@ -167,6 +167,41 @@ def test_c3_turbulence_full_size():
         assert du <= tol * gmax
 
 
+@pytest.mark.parametrize("L", [64, 128])
+def test_fp16s_turbulence_relative_l2(L):
+    """FP16S storage within 1e-3 relative L2 velocity error of the FP64
+    oracle: the C3 turbulence generator (k0 4, per-component u_rms 0.05,
+    seed 2024) at L^3, BGK + Smagorinsky (C_s 0.17, tau 0.51), after 100
+    calls -- inside FP32's own predictability horizon (FP32 storage is
+    checked on the same run at 1e-5)."""
+    import workloads
+
+    steps = 100
+    n = L ** 3
+    print(f"[fp16s l2] {L}^3, {steps} calls", flush=True)
+    ux, uy, uz = workloads.decaying_turbulence(L, k0=4.0, u_rms=0.05, seed=2024)
+    u0 = [ux.reshape(-1), uy.reshape(-1), uz.reshape(-1)]
+    desc = dict(lattice=lbmc.D3Q15, length=(L, L, L), collision_type=lbmc.BGK, tau=0.51, c_smag=0.17)
+    pyoracle.set_threads(min(16, os.cpu_count() or 1))
+    o = pyoracle.Oracle(**desc)
+    o.set_fields(geometry=np.zeros(n))
+    o.init_equilibrium(np.ones(n), u0)
+    o.run(0, steps, steps)
+    _, uo = o.get_macros()
+    o.close()
+    norm = float(np.linalg.norm(uo))
+    for storage, bar in ((lbmc.FP32, 1e-5), (lbmc.FP16S, 1e-3)):
+        lat = lbmc.Lattice(**desc, storage=storage)
+        lat.set_fields(geometry=np.zeros(n))
+        lat.init_equilibrium(np.ones(n), u0)
+        lat.run(0, steps, steps)
+        _, u = lat.get_macros()
+        lat.close()
+        rel = float(np.linalg.norm(u - uo)) / norm
+        print(f"[fp16s l2] {L}^3 {lbmc.STORAGE_NAMES[storage]}: relative L2 velocity error {rel:.3e}", flush=True)
+        assert rel <= bar
+
+
 def test_c5_ensemble_full_size():
     """The 64-member D2Q9 ensemble (11.45 M nodes; MRT and BGK members with
     Guo forcing, seeded discs) with FP64 storage in one batch, 5 calls from
