@@ -38,8 +38,15 @@ __global__ void init_static_kernel(typename St::S *f, long long npad, long long 
     }
 }
 
+// element of slot k (step_kernel.hpp slot_elem): the half-interleaved 16-bit
+// layout keeps slots k and k + hs in one word
+__device__ __forceinline__ long long elem_of(long long k, long long hs)
+{
+    return hs == 0 ? k : (k < hs ? 2 * k : 2 * (k - hs) + 1);
+}
+
 template <class L, class St>
-__global__ void init_eq_kernel(typename St::S *f, long long npad, long long n, const double *rho,
+__global__ void init_eq_kernel(typename St::S *f, long long npad, long long n, long long hs, const double *rho,
                                const double *ux, const double *uy, const double *uz)
 {
     using C = typename St::C;
@@ -52,26 +59,26 @@ __global__ void init_eq_kernel(typename St::S *f, long long npad, long long n, c
             constexpr int i = decltype(ii)::value;
             double v = feq[i];
             if constexpr (St::shifted) v = v - L::w[i];
-            f[(size_t)i * npad + k] = St::store((C)v);
+            f[(size_t)i * npad + elem_of(k, hs)] = St::store((C)v);
         });
     }
 }
 
 template <class L, class St>
-__global__ void f_to_double_kernel(const typename St::S *f, long long npad, int i, long long n,
-                                   double *out)
+__global__ void f_to_double_kernel(const typename St::S *f, long long npad, int i, long long first, long long n,
+                                   long long hs, double *out)
 {
     const double w = L::w[i];
     for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
          k += (long long)gridDim.x * blockDim.x) {
-        double v = (double)St::load(f[(size_t)i * npad + k]);
+        double v = (double)St::load(f[(size_t)i * npad + elem_of(first + k, hs)]);
         if constexpr (St::shifted) v = w + v;
         out[k] = v;
     }
 }
 
 template <class L, class St>
-__global__ void f_from_double_kernel(typename St::S *f, long long npad, int i, long long n,
+__global__ void f_from_double_kernel(typename St::S *f, long long npad, int i, long long n, long long hs,
                                      const double *in)
 {
     using C = typename St::C;
@@ -80,7 +87,7 @@ __global__ void f_from_double_kernel(typename St::S *f, long long npad, int i, l
          k += (long long)gridDim.x * blockDim.x) {
         double v = in[k];
         if constexpr (St::shifted) v = v - w;
-        f[(size_t)i * npad + k] = St::store((C)v);
+        f[(size_t)i * npad + elem_of(k, hs)] = St::store((C)v);
     }
 }
 
@@ -192,7 +199,7 @@ hipError_t launch_init_static(int lattice, int storage, void *f, long long npad,
     });
 }
 
-hipError_t launch_init_equilibrium(int lattice, int storage, void *f, long long npad, long long n,
+hipError_t launch_init_equilibrium(int lattice, int storage, void *f, long long npad, long long n, long long hs,
                                    const double *rho, const double *ux, const double *uy,
                                    const double *uz, hipStream_t s)
 {
@@ -200,31 +207,31 @@ hipError_t launch_init_equilibrium(int lattice, int storage, void *f, long long 
         using L = decltype(l);
         using St = decltype(st);
         hipLaunchKernelGGL((init_eq_kernel<L, St>), dim3(grid_for(n)), dim3(kAuxBlock), 0, s,
-                           (typename St::S *)f, npad, n, rho, ux, uy, uz);
+                           (typename St::S *)f, npad, n, hs, rho, ux, uy, uz);
         return hipGetLastError();
     });
 }
 
 hipError_t launch_f_to_double(int lattice, int storage, const void *f, long long npad, int i,
-                              long long n, double *out, hipStream_t s)
+                              long long first, long long n, long long hs, double *out, hipStream_t s)
 {
     return by_lattice_storage(lattice, storage, [&](auto l, auto st) {
         using L = decltype(l);
         using St = decltype(st);
         hipLaunchKernelGGL((f_to_double_kernel<L, St>), dim3(grid_for(n)), dim3(kAuxBlock), 0, s,
-                           (const typename St::S *)f, npad, i, n, out);
+                           (const typename St::S *)f, npad, i, first, n, hs, out);
         return hipGetLastError();
     });
 }
 
 hipError_t launch_f_from_double(int lattice, int storage, void *f, long long npad, int i,
-                                long long n, const double *in, hipStream_t s)
+                                long long n, long long hs, const double *in, hipStream_t s)
 {
     return by_lattice_storage(lattice, storage, [&](auto l, auto st) {
         using L = decltype(l);
         using St = decltype(st);
         hipLaunchKernelGGL((f_from_double_kernel<L, St>), dim3(grid_for(n)), dim3(kAuxBlock), 0, s,
-                           (typename St::S *)f, npad, i, n, in);
+                           (typename St::S *)f, npad, i, n, hs, in);
         return hipGetLastError();
     });
 }

@@ -31,14 +31,15 @@ hipError_t launch_sf_gather_bf16s(int lattice, const StepArgs<float> &a, const i
 // auxiliary kernels, aux.hip
 hipError_t launch_init_static(int lattice, int storage, void *f, long long npad, long long n,
                               hipStream_t s);
-hipError_t launch_init_equilibrium(int lattice, int storage, void *f, long long npad, long long n,
+// hs: the half-interleaved 16-bit layout's N / 2, or 0 (StepArgs::hs)
+hipError_t launch_init_equilibrium(int lattice, int storage, void *f, long long npad, long long n, long long hs,
                                    const double *rho, const double *ux, const double *uy,
                                    const double *uz, hipStream_t s);
-// direction i of f <-> double[n]
+// slots first .. first + n - 1 of direction i of f -> double[n]; all n slots <- double[n]
 hipError_t launch_f_to_double(int lattice, int storage, const void *f, long long npad, int i,
-                              long long n, double *out, hipStream_t s);
+                              long long first, long long n, long long hs, double *out, hipStream_t s);
 hipError_t launch_f_from_double(int lattice, int storage, void *f, long long npad, int i,
-                                long long n, const double *in, hipStream_t s);
+                                long long n, long long hs, const double *in, hipStream_t s);
 // E partials: one double per block; then the ordered final sum
 hipError_t launch_energy(int dim, int cdouble, const void *rho, const void *ux, const void *uy,
                          const void *uz, long long n, double *partials, int nblocks, double *out,

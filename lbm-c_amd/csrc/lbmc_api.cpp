@@ -308,6 +308,15 @@ lbmc_status lbmc_create(const lbmc_desc *desc, lbmc_ctx **out)
     // test hook: run the 64-bit-offset kernels on a small lattice
     if (const char *env = getenv("LBMC_FORCE_WIDE"))
         if (atoi(env) != 0) c->key.wide = 1;
+    // 16-bit storage with an even last axis (Lz; Ly on D2Q9) below the
+    // 4 GiB-per-array limit: slots k and k + N/2 share one 32-bit word, so
+    // the step moves two nodes per lane with aligned dword accesses on every
+    // call (step_half_kernel).  LBMC_HALF_LAYOUT=0 keeps slot k at element k.
+    if (c->sbytes == 2 && !c->key.wide && c->L[D - 1] % 2 == 0) {
+        c->hs = N / 2;
+        if (const char *env = getenv("LBMC_HALF_LAYOUT"))
+            if (atoi(env) == 0) c->hs = 0;
+    }
     choose_shape(c);
     if (const char *env = getenv("LBMC_GRAPH")) c->use_graph = atoi(env) != 0;
 
@@ -328,8 +337,9 @@ lbmc_status lbmc_create(const lbmc_desc *desc, lbmc_ctx **out)
         // (reserved LDS, which the step kernels do not use) raises it.
         int bpc = default_blocks_per_cu(c);
         if (const char *env = getenv("LBMC_STEP_BLOCKS_PER_CU")) bpc = atoi(env);
-        // two nodes per lane on calls with an even x shift (step_pair_kernel);
-        // LBMC_STEP_PAIRS=0 keeps one node per lane on every call
+        // two nodes per lane (step_half_kernel on every call in the
+        // half-interleaved layout, else step_pair_kernel on calls with an even
+        // x shift); LBMC_STEP_PAIRS=0 keeps one node per lane on every call
         c->key.pairs = 1;
         if (const char *env = getenv("LBMC_STEP_PAIRS")) c->key.pairs = atoi(env) != 0;
         int lds_cu = 0, lds_block = 0;
@@ -596,7 +606,7 @@ lbmc_status lbmc_init_equilibrium(lbmc_ctx *c, const double *rho, const double *
     if (e == hipSuccess) e = hipMemcpy(dr, rho, nb, hipMemcpyHostToDevice);
     for (int d = 0; d < c->D && e == hipSuccess; d++) e = hipMemcpy(du[d], u[d], nb, hipMemcpyHostToDevice);
     if (e == hipSuccess)
-        e = launch_init_equilibrium(c->desc.lattice, c->storage, c->f, c->npad, c->N, dr, du[0], du[1],
+        e = launch_init_equilibrium(c->desc.lattice, c->storage, c->f, c->npad, c->N, c->hs, dr, du[0], du[1],
                                     c->D > 2 ? du[2] : du[1], c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(dr);
@@ -693,7 +703,7 @@ lbmc_status lbmc_get_f(lbmc_ctx *c, double *const *f)
     hipError_t e = hipSuccess;
     for (int i = 0; i < c->Q && e == hipSuccess; i++) {
         if (!f[i]) continue;
-        e = launch_f_to_double(c->desc.lattice, c->storage, c->f, c->npad, i, c->N, tmp, c->stream);
+        e = launch_f_to_double(c->desc.lattice, c->storage, c->f, c->npad, i, 0, c->N, c->hs, tmp, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e == hipSuccess) e = hipMemcpy(f[i], tmp, sizeof(double) * (size_t)c->N, hipMemcpyDeviceToHost);
     }
@@ -745,10 +755,10 @@ lbmc_status lbmc_get_f_noalloc(lbmc_ctx *c, double *const *f)
         if (!f[i]) continue;
         for (long long k0 = 0; k0 < N; k0 += kNoallocChunk) {
             const long long cnt = std::min(kNoallocChunk, N - k0);
-            // slots k0 .. k0 + cnt - 1 of direction i: the same kernel as
-            // lbmc_get_f, on the population base advanced by k0 slots
-            const void *base = (const char *)c->f + (size_t)k0 * (size_t)c->sbytes;
-            HIP_TRY(c, launch_f_to_double(c->desc.lattice, c->storage, base, c->npad, i, cnt, buf, c->stream),
+            // slots k0 .. k0 + cnt - 1 of direction i, the kernel of lbmc_get_f
+            HIP_TRY(c,
+                    launch_f_to_double(c->desc.lattice, c->storage, c->f, c->npad, i, k0, cnt, c->hs, buf,
+                                       c->stream),
                     "get_f_noalloc launch");
             HIP_TRY(c, hipStreamSynchronize(c->stream), "get_f_noalloc sync");
             HIP_TRY(c, hipMemcpy(f[i] + k0, buf, sizeof(double) * (size_t)cnt, hipMemcpyDeviceToHost),
@@ -770,7 +780,7 @@ lbmc_status lbmc_set_f(lbmc_ctx *c, const double *const *f)
         if (!f[i]) continue;
         e = hipMemcpy(tmp, f[i], sizeof(double) * (size_t)c->N, hipMemcpyHostToDevice);
         if (e == hipSuccess)
-            e = launch_f_from_double(c->desc.lattice, c->storage, c->f, c->npad, i, c->N, tmp, c->stream);
+            e = launch_f_from_double(c->desc.lattice, c->storage, c->f, c->npad, i, c->N, c->hs, tmp, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     }
     (void)hipFree(tmp);

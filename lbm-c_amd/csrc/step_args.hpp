@@ -33,7 +33,8 @@ struct KernelKey {
     int closure;  // learned relaxation-time closure (BGK family, FP32 compute)
     int lds;      // dynamic LDS bytes per block of the step kernels (which use
                   //   none): caps the blocks resident per CU, 0 = no cap
-    int pairs;    // two nodes per lane on calls with an even x shift
+    int pairs;    // two nodes per lane: on every call in the half-interleaved 16-bit layout
+                  //   (StepArgs::hs), else on calls with an even x shift
 };
 
 enum : unsigned {
@@ -62,6 +63,10 @@ struct StepArgs {
     int lx, ly;              //   (q = (umulhi(n, m) + n) >> l, n < 2^31)
     int wide_rows;           // 64-bit-offset kernels with Lx % 64 == 0: every wave lies in
                              //   one x-row, so each direction's slot row is wave-uniform
+    unsigned hs;             // 16-bit storage in the half-interleaved layout: hs = nnodes / 2
+                             //   (the halves of the last axis); slot k < hs is element 2k and
+                             //   slot k + hs element 2k + 1 of the direction array, so the two
+                             //   share one 32-bit word.  0: slot k is element k
     // slot offsets of this call relative to the node index: for e = +1 along
     // an axis the slot is the node shifted by -s (or by L - s once wrapped),
     // for e = -1 by +s (or s - L); scaled by the axis stride

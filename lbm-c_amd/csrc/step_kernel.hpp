@@ -324,6 +324,14 @@ __device__ __forceinline__ void wide_row_slots(const StepArgs<typename St::C> &a
     });
 }
 
+// Element of slot k in a direction array: k, or in the half-interleaved
+// 16-bit layout (hs = N / 2 > 0) 2k for k < hs and 2(k - hs) + 1 above, so
+// that slots k and k + hs (nodes half the last axis apart) share a word.
+__device__ __forceinline__ unsigned slot_elem(unsigned k, unsigned hs)
+{
+    return hs == 0u ? k : (k < hs ? 2u * k : 2u * (k - hs) + 1u);
+}
+
 // index-shift gather of node n's populations (solver.cu:77-94) from the
 // slots n + dd[0][e_x+1] + dd[1][e_y+1] + dd[2][e_z+1]
 template <class L, class St, bool WIDE>
@@ -345,7 +353,8 @@ __device__ __forceinline__ void gather_node(const StepArgs<typename St::C> &a, i
             v[i] = St::load(ldg(fb + (size_t)i * a.npad + k));
         } else {
             const char *base = (const char *)(fb + (size_t)i * a.npad);
-            v[i] = St::load(ldg((const S *)(base + (unsigned)k * (unsigned)sizeof(S))));
+            const unsigned e = sizeof(S) == 2 ? slot_elem((unsigned)k, a.hs) : (unsigned)k;
+            v[i] = St::load(ldg((const S *)(base + e * (unsigned)sizeof(S))));
         }
     });
 }
@@ -394,7 +403,8 @@ __device__ __forceinline__ void scatter_node(const StepArgs<typename St::C> &a, 
             stg(fb + (size_t)i * a.npad + k, s);
         } else {
             char *base = (char *)(fb + (size_t)i * a.npad);
-            stg((S *)(base + (unsigned)k * (unsigned)sizeof(S)), s);
+            const unsigned e = sizeof(S) == 2 ? slot_elem((unsigned)k, a.hs) : (unsigned)k;
+            stg((S *)(base + e * (unsigned)sizeof(S)), s);
         }
     });
 }
@@ -514,6 +524,69 @@ __global__ __launch_bounds__(kBlock) void step_pair_kernel(const StepArgs<float>
     });
 }
 
+// Two nodes per lane in the half-interleaved 16-bit layout (StepArgs::hs =
+// N / 2 > 0), on every call: lane p takes nodes n0 = p and n1 = p + hs,
+// which sit half the last axis apart (z and z + Lz/2; y and y + Ly/2 on
+// D2Q9) at the same x and y.  The index shift moves both by the same
+// vector, so for every direction their slots are k0 and k0 +- hs: element
+// 2(k0 mod hs) and the element next to it, one aligned 32-bit word, with
+// node n0 in the low half when k0 < hs and in the high half otherwise.  One
+// dword load and one dword store per direction for both nodes, on odd and
+// even shifts alike; consecutive lanes are consecutive x, so a wave's
+// accesses for a direction are 64 consecutive words (one rotation point),
+// the FP32 step's pattern.  Each node takes step_kernel's update and
+// stochastic-rounding words: bit-identical results.  Direction arrays below
+// 4 GiB only.
+template <class L, class St, int FAM, bool GUO, bool SMAG, bool MFIX>
+__global__ __launch_bounds__(kBlock) void step_half_kernel(const StepArgs<float> a)
+{
+    using S = typename St::S;
+    static_assert(sizeof(S) == 2, "pairs of 16-bit slots");
+    constexpr int Q = L::Q;
+    // grid = ceil(hs / 256) blocks: p < 2^30
+    const unsigned p = blockIdx.x * (unsigned)kBlock + threadIdx.x;
+    const unsigned hs = a.hs;
+    if (p >= hs) return;
+    const int n0 = (int)p, n1 = (int)(p + hs);
+    int dd[3][3];
+    slot_offsets<L>(a, n0, dd);
+    float v0[Q], v1[Q];
+    unsigned lowmask = 0;  // bit i: node n0 holds the low half of direction i's word
+    constexpr bool PRIO = St::gather_prio && L::D == 3;
+    if constexpr (PRIO) __builtin_amdgcn_s_setprio(1);
+    sfor<0, Q>([&](auto ii) {
+        constexpr int i = decltype(ii)::value;
+        const unsigned k = (unsigned)(n0 + dd[0][L::ex[i] + 1] + dd[1][L::ey[i] + 1] + dd[2][L::ez[i] + 1]);
+        const bool low = k < hs;
+        const unsigned w = low ? k : k - hs;
+        const char *base = (const char *)((const S *)a.f + (size_t)i * a.npad);
+        const unsigned word = ldg((const unsigned *)(base + w * 4u));
+        const unsigned short lo = (unsigned short)(word & 0xffffu), hi = (unsigned short)(word >> 16);
+        v0[i] = St::load(__builtin_bit_cast(S, low ? lo : hi));
+        v1[i] = St::load(__builtin_bit_cast(S, low ? hi : lo));
+        lowmask |= low ? (1u << i) : 0u;
+    });
+    if constexpr (PRIO) {
+        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_setprio(0);
+    }
+    node_update<L, St, FAM, GUO, SMAG, MFIX>(a, n0, v0);
+    node_update<L, St, FAM, GUO, SMAG, MFIX>(a, n1, v1);
+    const unsigned ts = (unsigned)a.s[0] + (unsigned)a.L[0] * ((unsigned)a.s[1] + (unsigned)a.L[1] * (unsigned)a.s[2]);
+    unsigned r0 = sr_seed(n0, ts), r1 = sr_seed(n1, ts);
+    sfor<0, Q>([&](auto ii) {
+        constexpr int i = decltype(ii)::value;
+        const unsigned k = (unsigned)(n0 + dd[0][L::ex[i] + 1] + dd[1][L::ey[i] + 1] + dd[2][L::ez[i] + 1]);
+        const unsigned w = (lowmask >> i) & 1u ? k : k - hs;
+        const unsigned s0 = __builtin_bit_cast(unsigned short, St::store_sr(v0[i], r0));
+        const unsigned s1 = __builtin_bit_cast(unsigned short, St::store_sr(v1[i], r1));
+        r0 = sr_next(r0);
+        r1 = sr_next(r1);
+        char *base = (char *)((S *)a.f + (size_t)i * a.npad);
+        stg((unsigned *)(base + w * 4u), (lowmask >> i) & 1u ? (s0 | (s1 << 16)) : (s1 | (s0 << 16)));
+    });
+}
+
 // Gathers the post-stream populations of the neighbours of the sf nodes
 // before the step kernel overwrites any slot.  sfnodes[k] is a node index,
 // sfnb[k] its neighbour's index.
@@ -541,7 +614,8 @@ __global__ __launch_bounds__(kBlock) void sf_gather_kernel(const StepArgs<typena
         int zs = z - L::ez[i] * a.s[2];
         zs = zs < 0 ? zs + Lz : (zs >= Lz ? zs - Lz : zs);
         const int id = (zs * Ly + ys) * Lx + xs;
-        C v = St::load(fb[(size_t)i * a.npad + id]);
+        const unsigned e = sizeof(S) == 2 ? slot_elem((unsigned)id, a.hs) : (unsigned)id;
+        C v = St::load(fb[(size_t)i * a.npad + e]);
         if constexpr (St::shifted) v = v + C(L::w[i]);
         out[(size_t)k * Q + i] = v;
     });

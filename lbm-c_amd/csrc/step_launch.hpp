@@ -17,8 +17,15 @@ static hipError_t launch_one(const StepArgs<typename St::C> &a, int grid, hipStr
                              bool pairs)
 {
     if constexpr (sizeof(typename St::S) == 2) {
-        // even x shift on an even-length row: two nodes per lane
-        if (pairs && !wide && (a.L[0] & 1) == 0 && (a.s[0] & 1) == 0) {
+        // half-interleaved layout: two nodes per lane on every call
+        if (pairs && !wide && a.hs != 0) {
+            const int hgrid = (int)((a.hs + kBlock - 1) / kBlock);
+            hipLaunchKernelGGL((step_half_kernel<L, St, FAM, GUO, SMAG, MFIX>), dim3(hgrid), dim3(kBlock), lds, s,
+                               a);
+            return hipGetLastError();
+        }
+        // plain layout, even x shift on an even-length row: two nodes per lane
+        if (pairs && !wide && a.hs == 0 && (a.L[0] & 1) == 0 && (a.s[0] & 1) == 0) {
             const int pgrid = (a.nnodes / 2 + kBlock - 1) / kBlock;
             hipLaunchKernelGGL((step_pair_kernel<L, St, FAM, GUO, SMAG, MFIX>), dim3(pgrid), dim3(kBlock), lds, s,
                                a);

@@ -85,9 +85,10 @@ def test_batch_matches_single_lattices(storage):
 @pytest.mark.parametrize("storage", [lbmc.FP16S, lbmc.BF16S], ids=["fp16s", "bf16s"])
 @pytest.mark.parametrize("lattice", [lbmc.D2Q9, lbmc.D3Q15], ids=["d2q9", "d3q15"])
 def test_pair_steps_match_one_node_steps_bitwise(lattice, storage, t0):
-    """A 16-bit lattice stepped on its own takes two nodes per lane on calls
-    with an even x shift (every case here has an even Lx); a batch member
-    takes one node per lane on every call, with the same per-node update and
+    """A 16-bit lattice stepped on its own takes two nodes per lane: on every
+    call in the half-interleaved layout (last axis even), else on calls with
+    an even x shift (every case here has an even Lx); a batch member takes
+    one node per lane on every call, with the same per-node update and
     stochastic-rounding words: populations and macros bit-identical.  So
     is the same lattice with LBMC_STEP_PAIRS=0 (read at create: one node per
     lane on every call)."""
@@ -112,6 +113,75 @@ def test_pair_steps_match_one_node_steps_bitwise(lattice, storage, t0):
         lat.run(t0, steps, 7)
         assert np.array_equal(lat.get_f(), fs), c["name"]
         lat.close()
+
+
+def _run_env(c, storage, t0, steps, env):
+    saved = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        lat = cases.build(lbmc.Lattice, c, storage=storage)
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
+    lat.run(t0, steps, 7)
+    out = lat.get_f(), lat.get_macros()
+    lat.close()
+    return out
+
+
+@pytest.mark.parametrize("storage", [lbmc.FP16S, lbmc.BF16S], ids=["fp16s", "bf16s"])
+def test_half_layout_matches_plain_layout_bitwise(storage):
+    """16-bit lattices with an even last axis keep slots k and k + N/2 in one
+    32-bit word and step two nodes per lane on every call; with
+    LBMC_HALF_LAYOUT=0 (read at create) slot k stays at element k and pairs
+    run on even x shifts only; with LBMC_STEP_PAIRS=0 as well, one node per
+    lane.  Every case (sf faces, boundary codes and odd last axes included),
+    from t0 = 0 and from a large t0: populations (read back in slot order)
+    and macros bit-identical."""
+    for c in cases.all_cases():
+        for t0 in (0, 2 ** 31 + 5):
+            steps = min(c["steps"], 23)
+            print(f"[half layout] {c['name']} storage={storage} t0={t0}", flush=True)
+            fh, (rh, uh) = _run_env(c, storage, t0, steps, {})
+            for env in ({"LBMC_HALF_LAYOUT": "0"}, {"LBMC_STEP_PAIRS": "0"},
+                        {"LBMC_HALF_LAYOUT": "0", "LBMC_STEP_PAIRS": "0"}):
+                fp, (rp, up) = _run_env(c, storage, t0, steps, env)
+                assert np.array_equal(fh, fp), (c["name"], env)
+                assert np.array_equal(rh, rp) and np.array_equal(uh, up), (c["name"], env)
+
+
+def test_half_layout_set_f_round_trip():
+    """set_f then get_f returns the same populations in either 16-bit layout
+    (values that the storage holds exactly), and a half-layout lattice read
+    back equals a plain-layout lattice given the same populations."""
+    rng = np.random.default_rng(3)
+    for lattice, dims in ((lbmc.D3Q15, (12, 10, 8)), (lbmc.D2Q9, (30, 14))):
+        q = 15 if lattice == lbmc.D3Q15 else 9
+        n = int(np.prod(dims))
+        w = np.array([2 / 9] + [1 / 9] * 6 + [1 / 72] * 8) if q == 15 else \
+            np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+        g = np.round(rng.standard_normal((q, n)) * 64) / 2 ** 18   # exact in FP16S (2^12 g) and BF16S
+        f = w[:, None] + g
+        got = {}
+        for env in ({}, {"LBMC_HALF_LAYOUT": "0"}):
+            saved = os.environ.get("LBMC_HALF_LAYOUT")
+            os.environ.update(env)
+            try:
+                lat = lbmc.Lattice(lattice, dims, lbmc.BGK, tau=0.8, storage=lbmc.FP16S)
+            finally:
+                if env:
+                    if saved is None:
+                        del os.environ["LBMC_HALF_LAYOUT"]
+                    else:
+                        os.environ["LBMC_HALF_LAYOUT"] = saved
+            lat.set_fields(geometry=np.zeros(n))
+            lat.set_f(f)
+            got[len(env)] = lat.get_f()
+            lat.close()
+        assert np.array_equal(got[0], f) and np.array_equal(got[1], f)
 
 
 def test_batch_store_schedule():

@@ -74,9 +74,11 @@ def test_noalloc_matches_get_f_on_two_calls():
     assert np.array_equal(ref, out), f"differs at {np.argwhere(ref != out)[:5]}"
 
 
-@pytest.mark.parametrize("storage", [lbmc.FP64, lbmc.BF16S])
-def test_noalloc_other_storages(storage):
-    dims = (64, 48, 20)
+@pytest.mark.parametrize("storage,dims", [(lbmc.FP64, (64, 48, 20)), (lbmc.BF16S, (64, 48, 20)),
+                                          (lbmc.FP16S, (128, 128, 130))],
+                         ids=["fp64", "bf16s", "fp16s-3chunks"])
+def test_noalloc_other_storages(storage, dims):
+    # the 16-bit lattices (even last axis) are in the half-interleaved layout
     print(f"[no-alloc] D3Q15 {lbmc.STORAGE_NAMES[storage]} {dims}", flush=True)
     lat = _turbulent_d3q15(dims, storage, 13)
     ref, out = _read_both(lat)
