@@ -9,8 +9,10 @@ E(k) ~ k^4 exp(-2 (k/k0)^2), k0 = 4, per-component u_rms = 0.05, f = f_eq
 kernel over all 256^3 nodes, with the populations resident in HBM.
 
 Timing: after set-up, ~50 ms of a plain device copy (the roofline's
-bandwidth reference; it also lifts the GPU from the clocks an idle gap
-leaves), W untimed warm-up steps, then exactly K steps between HIP events
+bandwidth reference) and 100 calls of an identical twin lattice (the GPU's
+clocks leave the transient that follows any change of load; the measured
+lattice itself starts at t = 0), W untimed warm-up steps, then exactly K
+steps between HIP events
 recorded on the library's stream (the stream the kernel runs on), with a
 stream synchronisation before and after.  One process, one GPU.  Five
 further windows of K/5 steps (at least 10) on the same lattice give the
@@ -109,8 +111,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--trace-calls", type=int, default=0, metavar="N",
                     help="diagnostic instead of the benchmark: per-call step times (HIP events) of "
-                         "N calls of the workload, started after a 2 s idle gap and after the "
-                         "copy preload, FP32 and FP16S; prints one JSON line")
+                         "N calls of the workload, FP32 and FP16S: a fresh start after a 2 s idle "
+                         "gap, a fresh start after the copy preload, a settled lattice after a 2 s "
+                         "idle gap; prints one JSON line")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -131,40 +134,59 @@ def main():
     fields = tuple(v.reshape(-1) for v in fields)
 
     def preload():
-        """A plain 1 GiB copy (reads + writes 1 GiB each) for ~50 ms right
-        before a warm-up: the bandwidth reference of the roofline, and it
-        brings the GPU out of the idle that host-side set-up leaves, so that
-        the timed steps run at the clocks of a long run.  After an idle gap
-        the power controller lowers the shader clock for ~10-25 ms of load
-        (to ~1.8 GHz from ~2.4 GHz in per-dispatch counter samples), whatever
-        the lattice holds; a 20-step window started from idle times that
-        ramp, not the step."""
+        """A plain 1 GiB copy (reads + writes 1 GiB each) for ~50 ms: the
+        bandwidth reference of the roofline.  It also ends the idle that
+        host-side set-up leaves.  After an idle gap the shader clock dips for
+        ~10-40 calls of load (per-dispatch counter samples: ~2.2 GHz, down to
+        ~1.8 GHz, then ~2.45 GHz), whatever the lattice holds; a 20-step
+        window started from idle times that transient, not the step."""
         nbytes, reps = 1 << 30, 150
         ms_ = lbmc.copy_bandwidth(nbytes, reps)
         return 2.0 * nbytes * reps / (ms_ * 1e-3) / 1e9
 
     if args.trace_calls > 0:
-        # the same lattice state (a fresh start) timed call by call, once
-        # after the GPU idled for 2 s and once right after the preload
+        # call-by-call times of (1) a fresh start after the GPU idled for 2 s,
+        # (2) a fresh start right after the preload, (3) a lattice settled by
+        # 300 calls, after the GPU idled for 2 s: if (1) and (3) slow down
+        # alike and (2) does not, the slowdown follows the idle gap, not the
+        # lattice's state
         out = {"trace_calls": args.trace_calls, "unit": "us per call", "workload": f"{L}^3"}
         for st in (lbmc.FP32, lbmc.FP16S):
-            for start in ("idle", "preload"):
+            for start in ("idle", "preload", "settled_idle"):
                 tl = make_lattice(lbmc, L, st, fields)
-                if start == "idle":
-                    time.sleep(2.0)
-                else:
+                t0 = 0
+                if start == "settled_idle":
+                    tl.run(0, 300, 0)
+                    tl.sync()
+                    t0 = 300
+                if start == "preload":
                     preload()
-                per = [round(tl.time_steps(t, 1, 0) * 1e3, 1) for t in range(args.trace_calls)]
+                else:
+                    time.sleep(2.0)
+                per = [round(tl.time_steps(t0 + t, 1, 0) * 1e3, 1) for t in range(args.trace_calls)]
                 tl.close()
                 out[f"{lbmc.STORAGE_NAMES[st]}_{start}"] = per
-                log(f"bench: {lbmc.STORAGE_NAMES[st]} after {start}: " + " ".join(f"{v:.0f}" for v in per))
+                log(f"bench: {lbmc.STORAGE_NAMES[st]} {start}: " + " ".join(f"{v:.0f}" for v in per))
         print(json.dumps(out), flush=True)
         return
 
+    def steady(factory):
+        """A fresh lattice from factory() (t = 0, its state untouched) and an
+        identical twin that has just run 100 calls: the GPU then runs the
+        measured calls at the clocks of a long run of the same kernel.  A
+        change of load (idle, or another kernel) is followed by a clock
+        transient of ~10-40 calls that the copy preload alone only shortens
+        (bench.py --trace-calls).  The caller closes both after timing."""
+        lat = factory()
+        twin = factory()
+        twin.run(0, 100, 0)
+        twin.sync()
+        return lat, twin
+
     log(f"bench: FP32 lattice {L}^3, warmup {args.warmup}, timed {args.steps}")
-    lat = make_lattice(lbmc, L, lbmc.FP32, fields)
     copy_gbs = preload()
     log(f"bench: copy kernel {copy_gbs:.0f} GB/s")
+    lat, twin = steady(lambda: make_lattice(lbmc, L, lbmc.FP32, fields))
     lat.time_steps(0, args.warmup, 0)
     lat.sync()
     wall0 = time.perf_counter()
@@ -183,6 +205,7 @@ def main():
     rho, u = lat.get_macros()
     finite = bool(np.all(np.isfinite(rho)) and np.all(np.isfinite(u)))
     lat.close()
+    twin.close()
 
     ms_step = ms / args.steps
     mlups = N * args.steps / (ms * 1e-3) / 1e6
@@ -201,11 +224,11 @@ def main():
         # FP16S / BF16S: value-shifted 16-bit slots; FP64: the reference's own
         # arithmetic, bit for bit
         for st in (lbmc.FP16S, lbmc.BF16S, lbmc.FP64):
-            v = make_lattice(lbmc, L, st, fields)
+            v, vtwin = steady(lambda: make_lattice(lbmc, L, st, fields))
             vsteps = max(20, args.steps)
-            preload()
-            v.time_steps(0, max(5, args.warmup), 0)
+            v.time_steps(0, args.warmup, 0)
             vms = v.time_steps(args.warmup, vsteps, 0)
+            vtwin.close()
             vb = 2 * 15 * (8 if st == lbmc.FP64 else 2)
             variants[lbmc.STORAGE_NAMES[st]] = {
                 "MLUPS": round(N * vsteps / (vms * 1e-3) / 1e6, 1),
@@ -227,18 +250,32 @@ def main():
         isd[6] = 1e2
         model = lbmc.Mlp(closure.init_params(7), mu=mu, isd=isd, out_a=0.005, out_b=0.515)
         closure_line = {"mfma_flop_per_node": 2 * (16 * 64 + 64 * 64 + 64 * 16)}
-        for every in (1, 2, 4, 8):
+        # both steps over the same calls (a multiple of 8 calls from the
+        # first multiple of 8 after the warm-up), each after its own twin
+        c0 = (args.warmup + 7) // 8 * 8
+        csteps = max(32, (args.steps + 7) // 8 * 8)
+
+        def closure_lattice(every):
             c = make_lattice(lbmc, L, lbmc.FP32, fields)
             c.set_closure(model)
             c.set_closure_interval(every)
-            preload()
-            c.time_steps(0, max(8, args.warmup // 5), 0)
-            csteps = max(32, args.steps // 5)
-            cms = c.time_steps(args.warmup, csteps, 0)
+            return c
+
+        sm, smtwin = steady(lambda: make_lattice(lbmc, L, lbmc.FP32, fields))
+        sm.time_steps(0, c0, 0)
+        sms = sm.time_steps(c0, csteps, 0) / csteps
+        sm.close()
+        smtwin.close()
+        closure_line["smagorinsky_ms_per_step"] = round(sms, 5)
+        for every in (1, 2, 4, 8):
+            c, ctwin = steady(lambda: closure_lattice(every))
+            c.time_steps(0, c0, 0)
+            cms = c.time_steps(c0, csteps, 0)
             c.close()
+            ctwin.close()
             closure_line[f"every_{every}"] = {
                 "MLUPS": round(N * csteps / (cms * 1e-3) / 1e6, 1), "ms_per_step": round(cms / csteps, 5),
-                "vs_smagorinsky_step": round((cms / csteps) / ms_step, 3)}
+                "vs_smagorinsky_step": round((cms / csteps) / sms, 3)}
         model.close()
         log(f"bench: closure {closure_line}")
 
@@ -246,18 +283,15 @@ def main():
     if not args.no_ensemble:
         # 64 uneven D2Q9 channels with discs (seeded), FP32, one batched
         # launch per step, autotuned launch shape.  Members are MRT or
-        # BGK + learned closure (one shared network, He-initialised, centred
-        # on each member's own tau by its shift; fallback band [0.5005, 2]);
-        # timed with the network evaluated every call and every 8th call, and
-        # with plain BGK in place of BGK + learned.
+        # BGK + learned closure (one shared untrained network standing for
+        # the closure's cost, centred on each member's own tau by its shift;
+        # fallback band [0.5005, 2]); timed with the network evaluated every
+        # call and every 8th call, and with plain BGK in place of
+        # BGK + learned.
         specs = workloads.ensemble_d2q9(64, 0)
         EN = sum(m["length"][0] * m["length"][1] for m in specs)
-        # input scaling for channel flows (Pi / rho ~ 1e-4), output tau' =
-        # 0.55 + 0.01 y before the member's shift
-        ENS_TAU_REF = 0.55
-        ens_mu = np.array([0, 0, 0, 0, 0, 0, 1.0], dtype=np.float32)
-        ens_isd = np.array([1e4] * 6 + [100.0], dtype=np.float32)
-        model = lbmc.Mlp(closure.init_params(0), mu=ens_mu, isd=ens_isd, out_a=0.01, out_b=ENS_TAU_REF)
+        ens_p, ens_mu, ens_isd, ens_a, ENS_TAU_REF = workloads.ensemble_closure(0)
+        model = lbmc.Mlp(ens_p, mu=ens_mu, isd=ens_isd, out_a=ens_a, out_b=ENS_TAU_REF)
 
         def run_ensemble(every):
             lats = []
@@ -289,7 +323,7 @@ def main():
                     "frac": round(73 * EN / (ems / esteps * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
         ensemble_line = {"members": 64, "nodes": EN, "bytes_per_node": 2 * 9 * 4 + 1,
-                         "members_kind": "MRT or BGK + learned closure",
+                         "members_kind": "MRT or BGK + learned closure (untrained network: cost proxy)",
                          "closure_every_1": run_ensemble(1), "closure_every_8": run_ensemble(8),
                          "plain_bgk_mrt": run_ensemble(0)}
         model.close()

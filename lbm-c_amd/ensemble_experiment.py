@@ -4,14 +4,17 @@ aggregate throughput of the batched step and each member's error against a
 reference ensemble.
 
 Members are MRT or BGK; in the "learned" mix the BGK members use the learned
-relaxation-time closure (one shared, He-initialised network whose output
-tau' = 0.55 + 0.01 y is shifted to each member's tau; fallback band
-[0.5005, 2.0]), evaluated every call or every 8th call.
+relaxation-time closure (one shared, untrained He-initialised network,
+workloads.ensemble_closure, whose output tau' = 0.55 + 1e-3 y is shifted to
+each member's tau; fallback band [0.5005, 2.0]), evaluated every call or
+every 8th call: it stands for the closure's cost, not its physics.
 
   1. build the members, static start; autotune the launch shape on a
      throw-away copy, then time K steps: the plain mix at FP32, FP16S,
      BF16S and FP64 storage, the learned mix at FP32, FP16S and BF16S;
-  2. run the plain mix N steps (default 10,000) at every storage and compare
+  2. from a developed start (the plane Poiseuille profile at each member's
+     mean speed; --start static: f = w, as LBM-C's Init 0), run the plain
+     mix N steps (default 10,000) at every storage and compare
      u per member against FP64 storage (whose kernels match the FP64 CPU
      restatement of the reference bit for bit): relative L2 error over fluid
      nodes, and the member's Reynolds number from its mean velocity;
@@ -40,19 +43,17 @@ def log(msg):
     print(f"[ensemble {time.strftime('%H:%M:%S')}] {msg}", flush=True)
 
 
-TAU_REF = 0.55  # the shared network's own tau' centre
+TAU_REF = workloads.ENSEMBLE_TAU_REF  # the shared network's own tau' centre
 
 
 def make_model():
-    import closure
-
-    mu = np.array([0, 0, 0, 0, 0, 0, 1.0], dtype=np.float32)
-    isd = np.array([1e4] * 6 + [100.0], dtype=np.float32)
-    return lbmc.Mlp(closure.init_params(0), mu=mu, isd=isd, out_a=0.01, out_b=TAU_REF)
+    p, mu, isd, out_a, out_b = workloads.ensemble_closure(0)
+    return lbmc.Mlp(p, mu=mu, isd=isd, out_a=out_a, out_b=out_b)
 
 
-def build(specs, storage, model=None, every=1):
-    """The members; with a model, BGK members use the learned closure."""
+def build(specs, storage, model=None, every=1, developed=False):
+    """The members; with a model, BGK members use the learned closure; a
+    static start (f = w) or a developed one (workloads.ensemble_start)."""
     lats = []
     for m in specs:
         Lx, Ly = m["length"]
@@ -61,7 +62,11 @@ def build(specs, storage, model=None, every=1):
                            storage=storage)
         n = Lx * Ly
         lat.set_fields(geometry=m["geometry"].reshape(-1), force=[np.full(n, m["force"]), np.zeros(n)])
-        lat.init_static()
+        if developed:
+            rho, u = workloads.ensemble_start(m)
+            lat.init_equilibrium(rho, u)
+        else:
+            lat.init_static()
         if model is not None and coll == lbmc.BGK:
             lat.set_closure(model, 0.5005, 2.0)
             lat.set_closure_shift(m["tau"] - TAU_REF)
@@ -85,6 +90,9 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--steps", type=int, default=10000, help="steps of the error study")
     ap.add_argument("--time-steps", type=int, default=1000, help="timed steps of the throughput runs")
+    ap.add_argument("--start", choices=["developed", "static"], default="developed",
+                    help="initial state of the error study: the plane Poiseuille profile at each member's "
+                         "mean speed (the flows start at their target Reynolds number) or f = w")
     ap.add_argument("--out", type=str, default="ensemble_experiment.json")
     ap.add_argument("--plain-fp32-only", action="store_true",
                     help="only the plain-mix FP32 throughput run (short; for kernel traces)")
@@ -92,7 +100,7 @@ def main():
 
     specs = workloads.ensemble_d2q9(args.count, args.seed)
     N = sum(m["length"][0] * m["length"][1] for m in specs)
-    res = {"members": args.count, "seed": args.seed, "nodes": N, "steps": args.steps,
+    res = {"members": args.count, "seed": args.seed, "nodes": N, "steps": args.steps, "start": args.start,
            "sizes": [list(m["length"]) for m in specs], "Re_target": [m["Re"] for m in specs],
            "tau": [m["tau"] for m in specs], "collision": [m["collision"] for m in specs]}
     log(f"{args.count} members, {N} nodes")
@@ -141,7 +149,7 @@ def main():
     finals = {}
     for st in (lbmc.FP64, lbmc.FP32, lbmc.FP16S, lbmc.BF16S):
         name = lbmc.STORAGE_NAMES[st]
-        lats = build(specs, st)
+        lats = build(specs, st, developed=args.start == "developed")
         b = lbmc.Batch(lats)
         t0 = time.perf_counter()
         b.run(0, args.steps, args.steps)
@@ -162,7 +170,7 @@ def main():
     finals_l = {}
     for st in (lbmc.FP32, lbmc.FP16S, lbmc.BF16S):
         name = lbmc.STORAGE_NAMES[st]
-        lats = build(specs, st, model, 1)
+        lats = build(specs, st, model, 1, developed=args.start == "developed")
         b = lbmc.Batch(lats)
         b.run(0, args.steps, args.steps)
         finals_l[name] = [lat.get_macros()[1] for lat in lats]

@@ -10,6 +10,11 @@
   ensemble_d2q9        seeded ensemble of uneven D2Q9 channels with 1-6 discs
                        each, Reynolds number set through tau and the body
                        force (the batched-ensemble case)
+  ensemble_start       a member's developed start: the plane Poiseuille profile
+                       at its mean speed, zero in the solids
+  ensemble_closure     the ensemble's shared closure network: untrained
+                       (He-initialised) parameters and the scaling that keeps
+                       its tau' within a few 1e-3 of each member's tau
 
 All generators take an explicit seed and use numpy's PCG64 generator, so the
 fields (and the node masks built from them) are reproducible bit for bit.
@@ -94,7 +99,8 @@ def porous_discs(Lx, Ly, rmin=8, rmax=24, solid_fraction=0.30, seed=0, margin=No
 def ensemble_d2q9(count=64, seed=0, nx=(128, 1024), ny=(64, 512), re=(10.0, 1000.0), u_mean=0.04,
                   tau_min=0.505):
     """Member specifications of a batched ensemble: periodic-in-x channels
-    (solid rows y = 0 and y = ny-1) of seeded size, with 1-6 seeded
+    (solid rows y = 0 and y = ny-1) of seeded size (nx and ny uniform over
+    the integers of their ranges), with 1-6 seeded
     non-overlapping discs of radius in [ny/16, ny/8], driven by a uniform
     body force.  Re = u_mean D / nu with D the mean disc diameter, log-uniform
     in `re`; nu = u_mean D / Re gives tau = 1/2 + 3 nu (raised to tau_min, in
@@ -105,8 +111,8 @@ def ensemble_d2q9(count=64, seed=0, nx=(128, 1024), ny=(64, 512), re=(10.0, 1000
     rng = np.random.default_rng(seed)
     members = []
     for k in range(count):
-        Lx = int(rng.integers(nx[0] // 16, nx[1] // 16 + 1)) * 16
-        Ly = int(rng.integers(ny[0] // 16, ny[1] // 16 + 1)) * 16
+        Lx = int(rng.integers(nx[0], nx[1] + 1))
+        Ly = int(rng.integers(ny[0], ny[1] + 1))
         geo = channel(Lx, Ly)
         yy, xx = np.ogrid[0:Ly, 0:Lx]
         ndisc = int(rng.integers(1, 7))
@@ -135,7 +141,46 @@ def ensemble_d2q9(count=64, seed=0, nx=(128, 1024), ny=(64, 512), re=(10.0, 1000
         mrt = k % 2 == 0
         s = 1.0 / tau
         members.append(dict(
-            name=f"member{k}", length=(Lx, Ly), geometry=geo, discs=discs, Re=Re, tau=tau, force=F,
+            name=f"member{k}", length=(Lx, Ly), geometry=geo, discs=discs, Re=Re, tau=tau, force=F, u_mean=u,
             collision="MRT" if mrt else "BGK",
             rates=[0.0, 1.64, 1.54, s, 1.9, s, 1.9, s, s] if mrt else None))
     return members
+
+
+def ensemble_start(m, noise=0.0, rng=None):
+    """Initial (rho, [ux, uy]) of an ensemble member: the plane Poiseuille
+    profile u_x = 6 U eta (H - eta) / H^2 (mean U = the member's u_mean, walls
+    half a node outside the fluid rows, H = ny - 2) in the fluid, zero in the
+    solids, rho = 1; plus Gaussian noise of the given size in the fluid."""
+    Lx, Ly = m["length"]
+    H = Ly - 2
+    eta = np.arange(Ly) - 0.5
+    prof = np.clip(6.0 * m["u_mean"] * eta * (H - eta) / (H * H), 0.0, None)
+    ux = np.broadcast_to(prof[:, None], (Ly, Lx)).copy()
+    uy = np.zeros((Ly, Lx))
+    solid = m["geometry"] >= 1.0
+    ux[solid] = 0.0
+    if noise > 0.0:
+        ux += noise * rng.standard_normal((Ly, Lx)) * ~solid
+        uy += noise * rng.standard_normal((Ly, Lx)) * ~solid
+    return np.ones(Lx * Ly), [ux.reshape(-1), uy.reshape(-1)]
+
+
+# the ensemble network's own tau' centre; each member shifts it to its tau
+ENSEMBLE_TAU_REF = 0.55
+
+
+def ensemble_closure(seed=0):
+    """Parameters and scaling of the ensemble's shared closure network:
+    He-initialised flat parameters, inputs Pi_ab / rho scaled by 1e3 (their
+    size in these channels is ~1e-4-1e-3) and rho around 1 scaled by 100, and
+    tau' = 0.55 + 1e-3 y before each member's shift, so tau' stays within a
+    few 1e-3 of the member's tau.  The network is not trained: it stands for
+    the cost of a learned closure in the ensemble, not for its physics.
+    Returns (params, mu, isd, out_a, out_b)."""
+    import closure
+
+    p = closure.init_params(seed)
+    mu = np.array([0, 0, 0, 0, 0, 0, 1.0], dtype=np.float32)
+    isd = np.array([1e3] * 6 + [100.0], dtype=np.float32)
+    return p, mu, isd, 1e-3, ENSEMBLE_TAU_REF

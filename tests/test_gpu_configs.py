@@ -246,6 +246,112 @@ def test_c5_ensemble_full_size():
         o.close()
 
 
+def test_c5_ensemble_learned_members():
+    """The 64-member D2Q9 ensemble as the benchmark runs it: FP32 storage,
+    even members MRT, odd members BGK with the learned closure (one shared
+    He-initialised network, inputs scaled by 1e3, tau' = tau + 1e-3 y: each
+    member shifted to its own tau; fallback outside [0.5005, 2.0]), one
+    batch with the autotuned launch shape.  From a developed start (the plane Poiseuille
+    profile at the member's mean speed in the fluid, zero in the solids,
+    plus 1e-3 noise: non-trivial network inputs), 5 calls: sampled learned
+    members against the FP64 closure oracle (away from the reach of nodes
+    whose network tau comes within 1e-5 of a band edge, where FP32 inputs
+    may pick the other branch), sampled MRT members against the FP64
+    oracle, max|u - u_oracle| <= 2e-6 (|u| ~ 0.04)."""
+    import pyclosure
+    import pymlp
+    import workloads
+
+    specs = workloads.ensemble_d2q9(64, 0)
+    print("[c5 learned] 64 members, FP32, closure on BGK members", flush=True)
+    flat, mu, isd, OUT_A, TAU_REF = workloads.ensemble_closure(0)
+    model = lbmc.Mlp(flat, mu=mu, isd=isd, out_a=OUT_A, out_b=TAU_REF)
+    rng = np.random.default_rng(55)
+
+    inits = [workloads.ensemble_start(m, 1e-3, rng) for m in specs]
+    lats = []
+    for m, (rho, u) in zip(specs, inits):
+        Lx, Ly = m["length"]
+        n = Lx * Ly
+        lat = lbmc.Lattice(lbmc.D2Q9, (Lx, Ly), lbmc.MRT if m["collision"] == "MRT" else lbmc.BGK,
+                           forcing=True, tau=m["tau"], tau_mrt=m["rates"], storage=lbmc.FP32)
+        lat.set_fields(geometry=m["geometry"].reshape(-1), force=[np.full(n, m["force"]), np.zeros(n)])
+        lat.init_equilibrium(rho, u)
+        if m["collision"] != "MRT":
+            lat.set_closure(model, 0.5005, 2.0)
+            lat.set_closure_shift(m["tau"] - TAU_REF)
+        lats.append(lat)
+    b = lbmc.Batch(lats)
+    b.autotune(0, steps_per_trial=2)
+    shape = b.shape()
+    b.close()
+    for lat, (rho, u) in zip(lats, inits):
+        lat.init_equilibrium(rho, u)
+    b = lbmc.Batch(lats)
+    b.set_shape(*shape)
+    steps = 5
+    b.run(0, steps, steps)
+    got = [lat.get_macros() for lat in lats]
+    b.close()
+    for lat in lats:
+        lat.close()
+    model.close()
+
+    sizes = [m["length"][0] * m["length"][1] for m in specs]
+    learned = [k for k, m in enumerate(specs) if m["collision"] != "MRT"]
+    mrt = [k for k, m in enumerate(specs) if m["collision"] == "MRT"]
+    pick = lambda ks: sorted({min(ks, key=lambda k: sizes[k]), max(ks, key=lambda k: sizes[k]), ks[len(ks) // 2]})
+    params = pymlp.unflatten(flat.astype(np.float64))
+    for k in pick(learned):
+        m = specs[k]
+        rho0, u0 = inits[k]
+        co = pyclosure.ClosureOracle(pyoracle.D2Q9, m["length"], m["tau"], 0.0, params, mu, isd, OUT_A, TAU_REF,
+                                     0.5005, 2.0, force=[m["force"], 0.0], solid=m["geometry"].reshape(-1) >= 1.0,
+                                     shift=m["tau"] - TAU_REF)
+        co.init_equilibrium(rho0, u0)
+        # nodes whose network tau lies within 1e-5 of a band edge on some
+        # call may take the other branch on the device (FP32 inputs against
+        # FP64): leave out everything those nodes can reach in the calls
+        # that follow (one node per call along each axis)
+        Lx, Ly = m["length"]
+        near = np.zeros(Lx * Ly, dtype=bool)
+        for _ in range(steps):
+            co.step()
+            d = np.minimum(np.abs(co.tau_net - 0.5005), np.abs(co.tau_net - 2.0))
+            near |= d < 1e-5
+        ro, uo = co.rho, co.u
+        reach = near.reshape(Ly, Lx)
+        for _ in range(steps):
+            reach = reach | np.roll(reach, 1, 0) | np.roll(reach, -1, 0) | np.roll(reach, 1, 1) | np.roll(reach, -1, 1)
+        keep = ~reach.reshape(-1)
+        rg, ug = got[k]
+        du = float(np.max(np.abs(ug - uo)[:, keep]))
+        print(f"[c5 learned] member {k} {m['length']} tau {m['tau']:.4f}: max|du| {du:.2e} "
+              f"({int(near.sum())} nodes near a band edge, {1 - keep.mean():.4f} of the lattice left out), "
+              f"fallback share {co.fallback.mean():.3f}", flush=True)
+        assert keep.mean() > 0.9
+        assert du <= 2e-6
+        assert float(np.max(np.abs(rg - ro)[keep])) <= 2e-6
+    pyoracle.set_threads(min(16, os.cpu_count() or 1))
+    for k in pick(mrt):
+        m = specs[k]
+        Lx, Ly = m["length"]
+        n = Lx * Ly
+        rho0, u0 = inits[k]
+        o = pyoracle.Oracle(lattice=pyoracle.D2Q9, length=(Lx, Ly), collision_type=pyoracle.MRT, forcing=True,
+                            tau=m["tau"], tau_mrt=m["rates"])
+        o.set_fields(geometry=m["geometry"].reshape(-1), force=[np.full(n, m["force"]), np.zeros(n)])
+        o.init_equilibrium(rho0, u0)
+        o.run(0, steps, steps)
+        ro, uo = o.get_macros()
+        o.close()
+        rg, ug = got[k]
+        du = float(np.max(np.abs(ug - uo)))
+        print(f"[c5 learned] MRT member {k} {m['length']}: max|du| {du:.2e}", flush=True)
+        assert du <= 2e-6
+        assert float(np.max(np.abs(rg - ro))) <= 2e-6
+
+
 def test_max_size_wide_offsets_match_tiled_lattice():
     """A 1024^3 D3Q15 FP32 lattice (2^30 nodes: every direction array is
     4 GiB, so the step takes its 64-bit-offset kernels; 64 GB of
