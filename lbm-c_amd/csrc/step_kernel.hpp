@@ -498,8 +498,10 @@ __global__ __launch_bounds__(kBlock) void step_pair_kernel(const StepArgs<float>
         const int k = n + dd[0][L::ex[i] + 1] + dd[1][L::ey[i] + 1] + dd[2][L::ez[i] + 1];
         const char *base = (const char *)(fb + (size_t)i * a.npad);
         const unsigned w = ldg((const unsigned *)(base + (unsigned)k * 2u));
-        v0[i] = St::load(__builtin_bit_cast(S, (unsigned short)(w & 0xffffu)));
-        v1[i] = St::load(__builtin_bit_cast(S, (unsigned short)(w >> 16)));
+        typedef S s2 __attribute__((ext_vector_type(2)));
+        const s2 wv = __builtin_bit_cast(s2, w);
+        v0[i] = St::load(wv.x);
+        v1[i] = St::load(wv.y);
     });
     if constexpr (PRIO) {
         __builtin_amdgcn_sched_barrier(0);
@@ -513,13 +515,10 @@ __global__ __launch_bounds__(kBlock) void step_pair_kernel(const StepArgs<float>
     sfor<0, Q>([&](auto ii) {
         constexpr int i = decltype(ii)::value;
         const int k = n + dd[0][L::ex[i] + 1] + dd[1][L::ey[i] + 1] + dd[2][L::ez[i] + 1];
-        const S s0 = St::store_sr(v0[i], r0);
-        const S s1 = St::store_sr(v1[i], r1);
+        const unsigned w = St::store_sr_pair(v0[i], r0, v1[i], r1);
         r0 = sr_next(r0);
         r1 = sr_next(r1);
         char *base = (char *)(fw + (size_t)i * a.npad);
-        const unsigned w = (unsigned)__builtin_bit_cast(unsigned short, s0) |
-                           ((unsigned)__builtin_bit_cast(unsigned short, s1) << 16);
         stg((unsigned *)(base + (unsigned)k * 2u), w);
     });
 }
@@ -561,9 +560,12 @@ __global__ __launch_bounds__(kBlock) void step_half_kernel(const StepArgs<float>
         const unsigned w = low ? k : k - hs;
         const char *base = (const char *)((const S *)a.f + (size_t)i * a.npad);
         const unsigned word = ldg((const unsigned *)(base + w * 4u));
-        const unsigned short lo = (unsigned short)(word & 0xffffu), hi = (unsigned short)(word >> 16);
-        v0[i] = St::load(__builtin_bit_cast(S, low ? lo : hi));
-        v1[i] = St::load(__builtin_bit_cast(S, low ? hi : lo));
+        // both halves converted in place, then assigned to the nodes
+        typedef S s2 __attribute__((ext_vector_type(2)));
+        const s2 wv = __builtin_bit_cast(s2, word);
+        const float a0 = St::load(wv.x), a1 = St::load(wv.y);
+        v0[i] = low ? a0 : a1;
+        v1[i] = low ? a1 : a0;
         lowmask |= low ? (1u << i) : 0u;
     });
     if constexpr (PRIO) {
@@ -577,13 +579,15 @@ __global__ __launch_bounds__(kBlock) void step_half_kernel(const StepArgs<float>
     sfor<0, Q>([&](auto ii) {
         constexpr int i = decltype(ii)::value;
         const unsigned k = (unsigned)(n0 + dd[0][L::ex[i] + 1] + dd[1][L::ey[i] + 1] + dd[2][L::ez[i] + 1]);
-        const unsigned w = (lowmask >> i) & 1u ? k : k - hs;
-        const unsigned s0 = __builtin_bit_cast(unsigned short, St::store_sr(v0[i], r0));
-        const unsigned s1 = __builtin_bit_cast(unsigned short, St::store_sr(v1[i], r1));
+        const bool low = (lowmask >> i) & 1u;
+        const unsigned w = low ? k : k - hs;
+        // node n0's value in the low half, then the halves swapped where n0
+        // holds the high one
+        const unsigned word = St::store_sr_pair(v0[i], r0, v1[i], r1);
         r0 = sr_next(r0);
         r1 = sr_next(r1);
         char *base = (char *)((S *)a.f + (size_t)i * a.npad);
-        stg((unsigned *)(base + w * 4u), (lowmask >> i) & 1u ? (s0 | (s1 << 16)) : (s1 | (s0 << 16)));
+        stg((unsigned *)(base + w * 4u), low ? word : __builtin_amdgcn_alignbit(word, word, 16));
     });
 }
 

@@ -65,12 +65,23 @@ struct StFP16S {
     static constexpr bool gather_prio = true;
     static constexpr int id = ST_FP16S;
     static constexpr float kScale = 4096.f, kInv = 1.f / 4096.f;
-    __device__ static inline C load(S v) { return (float)v * kInv; }
+    // convert and scale in one mixed-precision FMA (the fp16 operand read in
+    // place from either half of a register); the zero addend makes a -0 slot
+    // load as +0, the same in every kernel
+    __device__ static inline C load(S v) { return __builtin_fmaf((float)v, kInv, 0.f); }
     __device__ static inline S store(C v) { return (_Float16)(v * kScale); }  // round to nearest even
     __device__ static inline S store_sr(C v, unsigned r)
     {
         typedef _Float16 h2 __attribute__((ext_vector_type(2)));
         return __builtin_amdgcn_cvt_sr_f16_f32((h2){0, 0}, v * kScale, r, false)[0];
+    }
+    // two stochastic-rounding stores into one 32-bit word: v0 low, v1 high
+    __device__ static inline unsigned store_sr_pair(C v0, unsigned r0, C v1, unsigned r1)
+    {
+        typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+        h2 w = __builtin_amdgcn_cvt_sr_f16_f32((h2){0, 0}, v0 * kScale, r0, false);
+        w = __builtin_amdgcn_cvt_sr_f16_f32(w, v1 * kScale, r1, true);
+        return __builtin_bit_cast(unsigned, w);
     }
 };
 
@@ -87,6 +98,14 @@ struct StBF16S {
     {
         typedef __bf16 b2 __attribute__((ext_vector_type(2)));
         return __builtin_amdgcn_cvt_sr_bf16_f32((b2){0, 0}, v, r, false)[0];
+    }
+    // two stochastic-rounding stores into one 32-bit word: v0 low, v1 high
+    __device__ static inline unsigned store_sr_pair(C v0, unsigned r0, C v1, unsigned r1)
+    {
+        typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+        b2 w = __builtin_amdgcn_cvt_sr_bf16_f32((b2){0, 0}, v0, r0, false);
+        w = __builtin_amdgcn_cvt_sr_bf16_f32(w, v1, r1, true);
+        return __builtin_bit_cast(unsigned, w);
     }
 };
 
