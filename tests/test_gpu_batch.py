@@ -134,19 +134,20 @@ def _run_env(c, storage, t0, steps, env):
 
 @pytest.mark.parametrize("storage", [lbmc.FP16S, lbmc.BF16S], ids=["fp16s", "bf16s"])
 def test_half_layout_matches_plain_layout_bitwise(storage):
-    """16-bit lattices with an even last axis keep slots k and k + N/2 in one
-    32-bit word and step two nodes per lane on every call; with
-    LBMC_HALF_LAYOUT=0 (read at create) slot k stays at element k and pairs
-    run on even x shifts only; with LBMC_STEP_PAIRS=0 as well, one node per
-    lane.  Every case (sf faces, boundary codes and odd last axes included),
-    from t0 = 0 and from a large t0: populations (read back in slot order)
-    and macros bit-identical."""
+    """16-bit lattices with an even last axis in the half-interleaved layout
+    (LBMC_HALF_LAYOUT=1, read at create; the default on D3Q15) keep slots k
+    and k + N/2 in one 32-bit word and step two nodes per lane on every call;
+    with LBMC_HALF_LAYOUT=0 slot k stays at element k and pairs run on even
+    x shifts only; with LBMC_STEP_PAIRS=0 as well, one node per lane.  Every
+    case (sf faces, boundary codes and odd last axes included), from t0 = 0
+    and from a large t0: populations (read back in slot order) and macros
+    bit-identical, and the default equals them."""
     for c in cases.all_cases():
         for t0 in (0, 2 ** 31 + 5):
             steps = min(c["steps"], 23)
             print(f"[half layout] {c['name']} storage={storage} t0={t0}", flush=True)
-            fh, (rh, uh) = _run_env(c, storage, t0, steps, {})
-            for env in ({"LBMC_HALF_LAYOUT": "0"}, {"LBMC_STEP_PAIRS": "0"},
+            fh, (rh, uh) = _run_env(c, storage, t0, steps, {"LBMC_HALF_LAYOUT": "1"})
+            for env in ({}, {"LBMC_HALF_LAYOUT": "0"}, {"LBMC_HALF_LAYOUT": "1", "LBMC_STEP_PAIRS": "0"},
                         {"LBMC_HALF_LAYOUT": "0", "LBMC_STEP_PAIRS": "0"}):
                 fp, (rp, up) = _run_env(c, storage, t0, steps, env)
                 assert np.array_equal(fh, fp), (c["name"], env)
@@ -165,21 +166,20 @@ def test_half_layout_set_f_round_trip():
             np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
         g = np.round(rng.standard_normal((q, n)) * 64) / 2 ** 18   # exact in FP16S (2^12 g) and BF16S
         f = w[:, None] + g
-        got = {}
-        for env in ({}, {"LBMC_HALF_LAYOUT": "0"}):
+        got = []
+        for flag in ("1", "0"):
             saved = os.environ.get("LBMC_HALF_LAYOUT")
-            os.environ.update(env)
+            os.environ["LBMC_HALF_LAYOUT"] = flag
             try:
                 lat = lbmc.Lattice(lattice, dims, lbmc.BGK, tau=0.8, storage=lbmc.FP16S)
             finally:
-                if env:
-                    if saved is None:
-                        del os.environ["LBMC_HALF_LAYOUT"]
-                    else:
-                        os.environ["LBMC_HALF_LAYOUT"] = saved
+                if saved is None:
+                    del os.environ["LBMC_HALF_LAYOUT"]
+                else:
+                    os.environ["LBMC_HALF_LAYOUT"] = saved
             lat.set_fields(geometry=np.zeros(n))
             lat.set_f(f)
-            got[len(env)] = lat.get_f()
+            got.append(lat.get_f())
             lat.close()
         assert np.array_equal(got[0], f) and np.array_equal(got[1], f)
 

@@ -78,7 +78,7 @@ def test_noalloc_matches_get_f_on_two_calls():
                                           (lbmc.FP16S, (128, 128, 130))],
                          ids=["fp64", "bf16s", "fp16s-3chunks"])
 def test_noalloc_other_storages(storage, dims):
-    # the 16-bit lattices (even last axis) are in the half-interleaved layout
+    # the 16-bit D3Q15 lattices (even last axis) are in the half-interleaved layout
     print(f"[no-alloc] D3Q15 {lbmc.STORAGE_NAMES[storage]} {dims}", flush=True)
     lat = _turbulent_d3q15(dims, storage, 13)
     ref, out = _read_both(lat)
