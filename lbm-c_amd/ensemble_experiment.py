@@ -4,10 +4,10 @@ aggregate throughput of the batched step and each member's error against a
 reference ensemble.
 
 Members are MRT or BGK; in the "learned" mix the BGK members use the learned
-relaxation-time closure (one shared, untrained He-initialised network,
-workloads.ensemble_closure, whose output tau' = 0.55 + 1e-3 y is shifted to
-each member's tau; fallback band [0.5005, 2.0]), evaluated every call or
-every 8th call: it stands for the closure's cost, not its physics.
+relaxation-time closure (one shared network trained on the ensemble's own
+box-filtered refined runs, workloads.ensemble_closure / ensemble_training.py,
+shifted to each member's tau; fallback band [0.5005, 2.0]), evaluated every
+call or every 8th call.
 
   1. build the members, static start; autotune the launch shape on a
      throw-away copy, then time K steps: the plain mix at FP32, FP16S,
@@ -43,12 +43,13 @@ def log(msg):
     print(f"[ensemble {time.strftime('%H:%M:%S')}] {msg}", flush=True)
 
 
-TAU_REF = workloads.ENSEMBLE_TAU_REF  # the shared network's own tau' centre
+# the shared network (trained: workloads.ensemble_closure); a member of
+# relaxation time tau takes the shift tau - TAU_REF
+_P, _MU, _ISD, _OUT_A, _OUT_B, TAU_REF = workloads.ensemble_closure()
 
 
 def make_model():
-    p, mu, isd, out_a, out_b = workloads.ensemble_closure(0)
-    return lbmc.Mlp(p, mu=mu, isd=isd, out_a=out_a, out_b=out_b)
+    return lbmc.Mlp(_P, mu=_MU, isd=_ISD, out_a=_OUT_A, out_b=_OUT_B)
 
 
 def build(specs, storage, model=None, every=1, developed=False):

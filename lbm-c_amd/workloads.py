@@ -12,14 +12,17 @@
                        force (the batched-ensemble case)
   ensemble_start       a member's developed start: the plane Poiseuille profile
                        at its mean speed, zero in the solids
-  ensemble_closure     the ensemble's shared closure network: untrained
-                       (He-initialised) parameters and the scaling that keeps
-                       its tau' within a few 1e-3 of each member's tau
+  ensemble_closure     the ensemble's shared closure network: trained on the
+                       ensemble's filtered refined runs (data/ensemble_closure.npz),
+                       or untrained with a scaling that keeps its tau' near
+                       each member's tau
 
 All generators take an explicit seed and use numpy's PCG64 generator, so the
 fields (and the node masks built from them) are reproducible bit for bit.
 """
 from __future__ import annotations
+
+import os
 
 import numpy as np
 
@@ -166,21 +169,30 @@ def ensemble_start(m, noise=0.0, rng=None):
     return np.ones(Lx * Ly), [ux.reshape(-1), uy.reshape(-1)]
 
 
-# the ensemble network's own tau' centre; each member shifts it to its tau
-ENSEMBLE_TAU_REF = 0.55
+# the trained ensemble network (lbm-c_amd/ensemble_training.py writes it)
+ENSEMBLE_CLOSURE_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "ensemble_closure.npz")
 
 
-def ensemble_closure(seed=0):
-    """Parameters and scaling of the ensemble's shared closure network:
-    He-initialised flat parameters, inputs Pi_ab / rho scaled by 1e3 (their
-    size in these channels is ~1e-4-1e-3) and rho around 1 scaled by 100, and
-    tau' = 0.55 + 1e-3 y before each member's shift, so tau' stays within a
-    few 1e-3 of the member's tau.  The network is not trained: it stands for
-    the cost of a learned closure in the ensemble, not for its physics.
-    Returns (params, mu, isd, out_a, out_b)."""
+def ensemble_closure(trained=True, seed=0):
+    """The ensemble's shared closure network: (params, mu, isd, out_a, out_b,
+    tau_ref); a member of relaxation time tau takes
+    lbmc_set_closure_shift(tau - tau_ref), so tau' = tau - tau_ref + out_b
+    + out_a y.
+
+    trained=True: the network trained a priori on box-filtered runs of the
+    ensemble's own channels refined by 2 (ensemble_training.py; it predicts
+    the eddy-viscosity correction tau' - tau, tau_ref = 0).  trained=False:
+    He-initialised parameters, inputs Pi_ab / rho scaled by 1e3 and rho
+    around 1 by 100, tau' = 0.55 + 1e-3 y before the shift (tau_ref = 0.55):
+    tau' stays within a few 1e-3 of the member's tau; it stands for the cost
+    of a network, not for its physics."""
+    if trained:
+        d = np.load(ENSEMBLE_CLOSURE_FILE)
+        return (d["params"].astype(np.float32), d["mu"].astype(np.float32), d["isd"].astype(np.float32),
+                float(d["out_a"]), float(d["out_b"]), 0.0)
     import closure
 
     p = closure.init_params(seed)
     mu = np.array([0, 0, 0, 0, 0, 0, 1.0], dtype=np.float32)
     isd = np.array([1e3] * 6 + [100.0], dtype=np.float32)
-    return p, mu, isd, 1e-3, ENSEMBLE_TAU_REF
+    return p, mu, isd, 1e-3, 0.55, 0.55

@@ -68,3 +68,67 @@ def test_init_params_layout():
     assert p.size == closure.NPARAM == 4737
     w1 = p[:512].reshape(64, 8)
     assert np.all(w1[:, 7] == 0) and np.all(p[closure.B2:closure.W3] == 0) and p[closure.B3] == 0
+
+
+# ---- 2-D pipeline of the ensemble's closure (lbm-c_amd/ensemble_training.py)
+
+def test_ensemble_training_filter_and_strain_2d():
+    import ensemble_training as et
+
+    a = np.arange(6 * 8, dtype=np.float64).reshape(6, 8)
+    assert et.box2(a, 2)[2, 3] == a[4:6, 6:8].mean()
+    L = 32
+    k = 2 * np.pi / L
+    y, x = np.meshgrid(np.arange(L), np.arange(L), indexing="ij")
+    S = et.strain2(np.stack([np.sin(k * y), np.zeros((L, L))]))
+    assert np.allclose(S[2], 0.5 * np.sin(k) * np.cos(k * y), atol=1e-12)
+    assert np.max(np.abs(S[0])) < 1e-12 and np.max(np.abs(S[1])) < 1e-12
+
+
+def test_ensemble_training_eddy_viscosity_2d():
+    # traceless 2-D strain (incompressible): tau = -2 nu S + c I gives nu
+    # exactly; backscatter clipped
+    rng = np.random.default_rng(2)
+    sxx = 1e-3 * rng.standard_normal((6, 7))
+    S = np.stack([sxx, -sxx, 1e-3 * rng.standard_normal((6, 7))])
+    c = rng.standard_normal((6, 7))
+    import ensemble_training as et
+
+    for nu in (0.004, -0.001):
+        ts = -2.0 * nu * S
+        ts[0] += c
+        ts[1] += c
+        assert np.allclose(et.nu_t2(ts, S), max(nu, 0.0), rtol=1e-9, atol=1e-15)
+
+
+def test_ensemble_training_refinement_keeps_reynolds_number():
+    import ensemble_training as et
+    import workloads
+
+    m = workloads.ensemble_d2q9(4, 0)[1]
+    mf = et.refine(m, 2)
+    assert mf["length"] == (2 * m["length"][0], 2 * m["length"][1])
+    assert np.array_equal(mf["geometry"][::2, ::2], m["geometry"])
+    nu, nuf = (m["tau"] - 0.5) / 3, (mf["tau"] - 0.5) / 3
+    # same physical viscosity in units of the coarse node and step: nu_f = r nu
+    assert np.isclose(nuf, 2 * nu) and np.isclose(mf["force"], m["force"] / 2)
+    # the start profile: zero in the solids, block means of the fine profile
+    # equal the coarse profile's shape (maximum near the centre line)
+    ux, uy = et.fine_start(m, 2, 0.0, None)
+    assert np.all(ux[mf["geometry"] >= 1] == 0) and np.all(uy == 0)
+    prof = et.box2(ux, 2)[:, 0]
+    assert abs(int(np.argmax(prof)) - (m["length"][1] - 1) / 2) <= 1
+
+
+def test_trained_ensemble_closure_file():
+    """The committed trained ensemble network loads without pickles, has the
+    network's parameter count and finite values, and predicts a correction
+    (tau_ref 0) of the size of the eddy viscosities it was trained on."""
+    import workloads
+
+    p, mu, isd, out_a, out_b, tau_ref = workloads.ensemble_closure()
+    assert p.shape == (4737,) and mu.shape == (7,) and isd.shape == (7,)
+    assert np.all(np.isfinite(p)) and np.all(np.isfinite(mu)) and np.all(isd > 0)
+    assert tau_ref == 0.0 and 0.0 < out_a < 1e-2 and 0.0 <= out_b < 1e-2
+    p0, _, _, a0, b0, r0 = workloads.ensemble_closure(trained=False)
+    assert p0.shape == (4737,) and (a0, b0, r0) == (1e-3, 0.55, 0.55)

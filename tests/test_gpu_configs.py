@@ -264,8 +264,10 @@ def test_c5_ensemble_learned_members():
 
     specs = workloads.ensemble_d2q9(64, 0)
     print("[c5 learned] 64 members, FP32, closure on BGK members", flush=True)
-    flat, mu, isd, OUT_A, TAU_REF = workloads.ensemble_closure(0)
-    model = lbmc.Mlp(flat, mu=mu, isd=isd, out_a=OUT_A, out_b=TAU_REF)
+    # the untrained network with its 1e-3 output scale: a discriminating
+    # check of the evaluation (the trained one's corrections are ~1e-5)
+    flat, mu, isd, OUT_A, OUT_B, TAU_REF = workloads.ensemble_closure(trained=False)
+    model = lbmc.Mlp(flat, mu=mu, isd=isd, out_a=OUT_A, out_b=OUT_B)
     rng = np.random.default_rng(55)
 
     inits = [workloads.ensemble_start(m, 1e-3, rng) for m in specs]
@@ -305,7 +307,7 @@ def test_c5_ensemble_learned_members():
     for k in pick(learned):
         m = specs[k]
         rho0, u0 = inits[k]
-        co = pyclosure.ClosureOracle(pyoracle.D2Q9, m["length"], m["tau"], 0.0, params, mu, isd, OUT_A, TAU_REF,
+        co = pyclosure.ClosureOracle(pyoracle.D2Q9, m["length"], m["tau"], 0.0, params, mu, isd, OUT_A, OUT_B,
                                      0.5005, 2.0, force=[m["force"], 0.0], solid=m["geometry"].reshape(-1) >= 1.0,
                                      shift=m["tau"] - TAU_REF)
         co.init_equilibrium(rho0, u0)
