@@ -112,12 +112,12 @@ __global__ __launch_bounds__(kBlock, 4) void batch_eval_kernel(const BatchMember
                                                                const int *__restrict__ any_eval)
 {
     if (*any_eval == 0) return;
-    __shared__ MlpInfer w;
+    __shared__ MlpEval w;
     __shared__ mlp_v8b xb[kBlock];
     {
-        const uint4 *src = (const uint4 *)mlp;
+        const uint4 *src = (const uint4 *)&((const MlpImage *)mlp)->ev;
         uint4 *dst = (uint4 *)&w;
-        for (int k = (int)threadIdx.x; k < (int)(sizeof(MlpInfer) / 16); k += kBlock) dst[k] = src[k];
+        for (int k = (int)threadIdx.x; k < (int)(sizeof(MlpEval) / 16); k += kBlock) dst[k] = src[k];
     }
     __syncthreads();
     mlp_v8b *xw = xb + (threadIdx.x & ~63u);

@@ -2,7 +2,7 @@
 // the Smagorinsky hook of bgk_collision (LBM-C/collision.cu:60-62,445-463).
 //
 // Same gather / update / scatter as step_kernel, with three differences:
-//  * the network's weights (16.5 KB of operands) are staged once per block
+//  * the network's weights (9.7 KB of operands) are staged once per block
 //    into LDS, so blocks loop over 256-node chunks (grid-stride) instead of
 //    one block per chunk;
 //  * every node of a wave evaluates the network together, in uniform control
@@ -23,7 +23,7 @@ namespace lbmc {
 
 template <class L, class St, bool GUO, bool RECORD>
 __device__ __forceinline__ void closure_update(const StepArgs<float> &a, int n, bool live, float (&v)[L::Q],
-                                               const MlpInfer &w, mlp_v8b *xw)
+                                               const MlpEval &w, mlp_v8b *xw)
 {
     constexpr int D = L::D;
     const unsigned m = a.all_fluid ? 0u : ldg(a.mask + n);
@@ -38,7 +38,7 @@ __device__ __forceinline__ void closure_update(const StepArgs<float> &a, int n, 
     fast_prepare<L, St::shifted, GUO>(v, F, st);
     float x[7];
     fast_closure_features<L, St::shifted>(st, x);
-    const float y = mlp_wave_infer(w, xw, mlp_input(w.sc, x));
+    const float y = mlp_wave_eval(w, xw, mlp_input(w.sc, x));
     if (!live) return;
     if (m == 0) {
         float t = w.sc.out_a * y + w.sc.out_b + a.closure_shift;
@@ -93,12 +93,12 @@ __device__ __forceinline__ void closure_reuse_node(const StepArgs<float> &a, int
 template <class L, class St, bool GUO, bool RECORD, bool WIDE>
 __global__ __launch_bounds__(kBlock, 4) void closure_step_kernel(const StepArgs<float> a)
 {
-    __shared__ MlpInfer w;
+    __shared__ MlpEval w;
     __shared__ mlp_v8b xb[kBlock];
     {
-        const uint4 *src = (const uint4 *)a.mlp;
+        const uint4 *src = (const uint4 *)&((const MlpImage *)a.mlp)->ev;
         uint4 *dst = (uint4 *)&w;
-        for (int k = (int)threadIdx.x; k < (int)(sizeof(MlpInfer) / 16); k += kBlock) dst[k] = src[k];
+        for (int k = (int)threadIdx.x; k < (int)(sizeof(MlpEval) / 16); k += kBlock) dst[k] = src[k];
     }
     __syncthreads();
     mlp_v8b *xw = xb + (threadIdx.x & ~63u);

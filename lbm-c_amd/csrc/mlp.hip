@@ -28,8 +28,10 @@ namespace lbmc {
 
 __device__ __forceinline__ __bf16 bf16_of(float x) { return __builtin_bit_cast(__bf16, bf16_bits(x)); }
 
-// Operand image from the flat FP32 parameters.  31 blocks of 64 lanes, one
+// Operand image from the flat FP32 parameters.  43 blocks of 64 lanes, one
 // block per operand array.
+constexpr int MLP_IMAGE_BLOCKS = 43;
+
 __global__ __launch_bounds__(64) void mlp_image_kernel(const float *p, MlpScalars sc, MlpImage *img)
 {
     const int b = (int)blockIdx.x;
@@ -66,10 +68,32 @@ __global__ __launch_bounds__(64) void mlp_image_kernel(const float *p, MlpScalar
         for (int j = 0; j < 4; j++)
             v[j] = __builtin_bit_cast(float, (unsigned)bf16_bits(w3[16 * h + 4 * g + j]) << 16);
         img->w3d[h][l] = v;
+    } else if (b < 38) {
+        // evaluation image (32x32 products)
+        const int m = (b - 30) >> 2, s = (b - 30) & 3, h = l >> 5, c = l & 31;
+        mlp_v8b v;
+        for (int j = 0; j < 8; j++) v[j] = bf16_of(W2[(32 * m + c) * 64 + mlp_hid32(s, 8 * h + j)]);
+        img->ev.w2[m][s][l] = v;
+    } else if (b < 40) {
+        const int m = b - 38, h = l >> 5, c = l & 31;
+        mlp_v4s v;
+        for (int j = 0; j < 4; j++) v[j] = (short)bf16_bits(W1[(32 * m + c) * 8 + 4 * h + j]);
+        img->ev.w1[m][l] = v;
+    } else if (b == 40) {
+        const int m = l >> 5, h = (l >> 4) & 1, e = l & 15;
+        img->ev.b2[m][h][e] = b2[32 * m + mlp_row32(h, e)];
+    } else if (b == 41) {
+        if (l < 32) {
+            const int h = l >> 4, m = (l >> 3) & 1, q = l & 7;
+            const unsigned lo = bf16_bits(w3[32 * m + mlp_row32(h, 2 * q)]);
+            const unsigned hi = bf16_bits(w3[32 * m + mlp_row32(h, 2 * q + 1)]);
+            img->ev.w3[h][m][q] = lo | (hi << 16);
+        }
     } else if (l == 0) {
         MlpScalars s = sc;
         s.b3 = p[MLP_B3];
         img->inf.sc = s;
+        img->ev.sc = s;
     }
 }
 
@@ -77,19 +101,19 @@ __global__ __launch_bounds__(64) void mlp_image_kernel(const float *p, MlpScalar
 __global__ __launch_bounds__(256) void mlp_forward_kernel(const MlpImage *img, const float *feat, long long S,
                                                           float *y)
 {
-    __shared__ MlpInfer w;
+    __shared__ MlpEval w;
     __shared__ mlp_v8b xb[256];
     {
-        const uint4 *src = (const uint4 *)&img->inf;
+        const uint4 *src = (const uint4 *)&img->ev;
         uint4 *dst = (uint4 *)&w;
-        for (int k = (int)threadIdx.x; k < (int)(sizeof(MlpInfer) / 16); k += 256) dst[k] = src[k];
+        for (int k = (int)threadIdx.x; k < (int)(sizeof(MlpEval) / 16); k += 256) dst[k] = src[k];
     }
     __syncthreads();
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     const bool valid = i < S;
     float f[7];
     for (int k = 0; k < 7; k++) f[k] = valid ? feat[i * 7 + k] : 0.f;
-    const float out = mlp_wave_infer(w, xb + (threadIdx.x & ~63u), mlp_input(w.sc, f));
+    const float out = mlp_wave_eval(w, xb + (threadIdx.x & ~63u), mlp_input(w.sc, f));
     if (valid) y[i] = out;
 }
 
@@ -300,7 +324,7 @@ lbmc_status mfail(lbmc_mlp *m, lbmc_status st, const char *fmt, ...)
 
 lbmc_status rebuild_image(lbmc_mlp *m)
 {
-    hipLaunchKernelGGL(mlp_image_kernel, dim3(31), dim3(64), 0, m->stream, m->params, m->sc, m->img);
+    hipLaunchKernelGGL(mlp_image_kernel, dim3(MLP_IMAGE_BLOCKS), dim3(64), 0, m->stream, m->params, m->sc, m->img);
     MLP_HIP(m, hipGetLastError());
     return LBMC_OK;
 }
@@ -383,7 +407,7 @@ lbmc_status mlp_build_image(const lbmc_mlp *m, void *dst, int device, double lo,
     MlpScalars sc = m->sc;
     sc.lo = (float)lo;
     sc.hi = (float)hi;
-    hipLaunchKernelGGL(mlp_image_kernel, dim3(31), dim3(64), 0, s, m->params, sc, (MlpImage *)dst);
+    hipLaunchKernelGGL(mlp_image_kernel, dim3(MLP_IMAGE_BLOCKS), dim3(64), 0, s, m->params, sc, (MlpImage *)dst);
     if (hipGetLastError() != hipSuccess) return LBMC_ERR_DEVICE;
     return LBMC_OK;
 }

@@ -4,8 +4,11 @@
 // called from bgk_collision at :60-62): tau' = out_a * y + out_b, and the
 // Smagorinsky tau is used wherever tau' leaves the band [lo, hi].
 //
-// Arithmetic: bf16 operands, FP32 accumulation (MFMA 16x16x16 bf16 for the
-// first layer, 16x16x32 bf16 for the others); biases and derivatives in FP32.
+// Arithmetic: bf16 operands, FP32 accumulation; biases and derivatives in
+// FP32.  Training (mlp_tile_forward and the backward pass) uses 16x16 tiles
+// (MFMA 16x16x16 bf16 for the first layer, 16x16x32 bf16 for the others);
+// evaluation in the closure steps (mlp_wave_eval) uses 32x32 tiles (32x32x8
+// and 32x32x16 bf16, the output layer as bf16 dot products).
 // GELU(z) = z Phi(z), Phi from an odd degree-9 polynomial (least-squares fit
 // of the normal CDF on [-3.5, 3.5], |error| <= 8.6e-4 in exact arithmetic),
 // clamped to [0, 1].  The polynomial runs two activations at a time in packed
@@ -17,8 +20,8 @@
 // the input needs no clamp; |GELU error| <= 5.2e-3, below the bf16 half-ulp
 // of the result (2^-9 relative) that follows.
 //
-// Wave mapping (64 lanes, lane l = 16 g + r): a wave evaluates 64 samples
-// as four tiles of 16.  Each layer is computed transposed, H = W X, with
+// Training's wave mapping (64 lanes, lane l = 16 g + r): a wave takes 64
+// samples as four tiles of 16 (mlp_wave_eval describes the evaluation's).  Each layer is computed transposed, H = W X, with
 // weights as the A operand and samples as the columns of B, so the MFMA
 // result layout of one layer (lane holds rows 4g..4g+3 of column r) is the
 // B-operand layout of the next: no data moves between lanes inside the
@@ -67,15 +70,42 @@ struct MlpInfer {
     MlpScalars sc;
 };
 
+typedef float mlp_v16f __attribute__((ext_vector_type(16)));
+
+// Row of accumulator element e held by lane half h (lanes 32h..32h+31) in a
+// 32x32 matrix-core result: column = lane & 31.
+__host__ __device__ constexpr int mlp_row32(int h, int e) { return 8 * (e >> 2) + 4 * h + (e & 3); }
+// Hidden unit at K position k (lane half k >> 3, element k & 7) of K-step s
+// of the evaluation's second layer: the order in which the first layer's
+// result (two 32-row tiles) already sits in the lanes.
+__host__ __device__ constexpr int mlp_hid32(int s, int k)
+{
+    return 16 * s + 8 * ((k & 7) >> 2) + 4 * (k >> 3) + (k & 3);
+}
+
+// Operand-ordered weights of the closure's evaluation (mlp_wave_eval):
+// 32x32 products, lane l = 32h + c.
+struct MlpEval {
+    mlp_v8b w2[2][4][64];  // A of layer 2, row tile m, K-step s: W2[32m + c][hid32(s, 8h + j)]
+    mlp_v4s w1[2][64];     // A of layer 1, row tile m: W1[32m + c][4h + j]
+    mlp_v16f b2[2][2];     // layer-2 accumulator start, row tile m, lane half h: b2[32m + row32(h, e)]
+    unsigned w3[2][2][8];  // bf16 pairs (low, high) of w3[32m + row32(h, 2q)], w3[32m + row32(h, 2q + 1)], [h][m][q]
+    MlpScalars sc;
+};
+
 struct MlpImage {
     MlpInfer inf;
     // backward only
     mlp_v8b w2t[4][2][64];  // A of W2^T, row tile h, chunk c: W2[hid(c, g, j)][16h + r]
     mlp_v4f w3d[4][64];     // w3[16h + 4g + j] as rounded to bf16, in FP32
+    // evaluation (closure steps and lbmc_mlp_forward)
+    MlpEval ev;
 };
 
 static_assert(sizeof(MlpInfer) % 16 == 0, "MlpInfer is copied in 16-byte units");
+static_assert(sizeof(MlpEval) % 16 == 0, "MlpEval is copied in 16-byte units");
 static_assert(sizeof(MlpImage) % 16 == 0, "MlpImage is copied in 16-byte units");
+static_assert(offsetof(MlpImage, ev) % 64 == 0, "MlpEval holds 64-byte vectors");
 
 // GELU coefficients: Phi(x) - 1/2 = x (c1 + t (c3 + t (c5 + t (c7 + t c9)))), t = x^2
 constexpr float GELU_C1 = 0.3964735269546509f;
@@ -221,21 +251,109 @@ __device__ __forceinline__ void mlp_tile_forward(const MlpInfer &w, const mlp_v8
     o.y = yv[0];
 }
 
-// Network output for the lane's own sample (input row xin).  xb: the wave's
-// 64-entry LDS staging area.  Called by all 64 lanes together.
-__device__ __forceinline__ float mlp_wave_infer(const MlpInfer &w, mlp_v8b *xb, mlp_v8b xin)
+__device__ __forceinline__ mlp_v16f mfma_32k8(mlp_v4s a, mlp_v4s b)
 {
-    const int l = (int)(threadIdx.x & 63);
+    return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, mlp_v16f{}, 0, 0, 0);
+}
+
+__device__ __forceinline__ mlp_v16f mfma_32k16(mlp_v8b a, mlp_v8b b, mlp_v16f c)
+{
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+typedef _Float16 mlp_h8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ mlp_h8 mlp_h8_splat(float c)
+{
+    const _Float16 h = (_Float16)c;
+    return mlp_h8{h, h, h, h, h, h, h, h};
+}
+
+// gelu_pair on z[e0 .. e0 + 7] (four pairs whose packed operations issue side
+// by side), as the bf16 operand of the next product, element j = z[e0 + j]
+__device__ __forceinline__ mlp_v8b gelu_oct(const mlp_v16f &z, int e0)
+{
+    mlp_h8 x;
+#pragma unroll
+    for (int j = 0; j < 8; j++) x[j] = (_Float16)z[e0 + j];
+    const mlp_h8 t = x * x;
+    mlp_h8 p = __builtin_elementwise_fma(t, mlp_h8_splat(GELU_C9), mlp_h8_splat(GELU_C7));
+    p = __builtin_elementwise_fma(t, p, mlp_h8_splat(GELU_C5));
+    p = __builtin_elementwise_fma(t, p, mlp_h8_splat(GELU_C3));
+    p = __builtin_elementwise_fma(t, p, mlp_h8_splat(GELU_C1));
+    mlp_h8 s = __builtin_elementwise_fma(x, p, mlp_h8_splat(0.5f));
+    s = __builtin_elementwise_min(__builtin_elementwise_max(s, mlp_h8_splat(0.f)), mlp_h8_splat(1.f));
+    mlp_v8b r;
+#pragma unroll
+    for (int j = 0; j < 8; j++) r[j] = (__bf16)__builtin_fmaf(z[e0 + j], (float)s[j], 0.f);
+    return r;
+}
+
+// sum over the four bf16 pairs of a and b, added to c in FP32
+__device__ __forceinline__ float dot8(mlp_v8b a, uint4 b, float c)
+{
+    const uint4 ua = __builtin_bit_cast(uint4, a);
+    c = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(mlp_b2, ua.x), __builtin_bit_cast(mlp_b2, b.x), c, false);
+    c = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(mlp_b2, ua.y), __builtin_bit_cast(mlp_b2, b.y), c, false);
+    c = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(mlp_b2, ua.z), __builtin_bit_cast(mlp_b2, b.z), c, false);
+    return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(mlp_b2, ua.w), __builtin_bit_cast(mlp_b2, b.w), c, false);
+}
+
+// Network output for the lane's own sample (input row xin), from the
+// evaluation image.  xb: the wave's 64-entry LDS staging area.  Called by all
+// 64 lanes together.
+//
+// The wave's 64 samples are two tiles of 32 (lane l = 32h + c holds column c
+// of a tile), each layer again computed transposed with the weights as A:
+// layer 1 is two 32x32x8 products (K = the 7 inputs and the bias input),
+// layer 2 two row tiles of four 32x32x16 products whose K order mlp_hid32 is
+// the order in which the lanes hold layer 1's GELUs, and the output layer a
+// dot product of each lane's 32 layer-2 GELUs with w3 (bf16 pairs, FP32 sum)
+// plus the other lane half's.  Against the 16x16 tiles of mlp_tile_forward
+// (training) this skips the zero K rows of layer 1 and the 15 unused rows of
+// the output layer's product, a quarter of the matrix-core work, and reads
+// half the weight operands per sample; only the FP32 summation order differs.
+__device__ __forceinline__ float mlp_wave_eval(const MlpEval &w, mlp_v8b *xb, mlp_v8b xin)
+{
+    const int l = (int)(threadIdx.x & 63), h = l >> 5, c = l & 31;
     xb[l] = xin;
     wave_sync();
+    // opaque lane and half indices: the operand reads stay inside the tile
+    // loop instead of holding ~90 registers across it
+    int lw = l, hw = h;
+    asm volatile("" : "+v"(lw), "+v"(hw));
     float y = 0.f;
-    // tiles one after another: unrolling them lets the scheduler interleave
-    // four tiles' operands and halves the occupancy
 #pragma unroll 1
-    for (int T = 0; T < 4; T++) {
-        MlpTile o;
-        mlp_tile_forward<false>(w, xb, T, l, o);
-        if ((l >> 4) == T) y = o.y;
+    for (int t = 0; t < 2; t++) {
+        // layer-1 B operand: inputs 4h..4h+3 of sample 32t + c
+        const mlp_v4s xop = ((const mlp_v4s *)&xb[32 * t + c])[h];
+        // one 32-row tile at a time, phases fenced from the scheduler: the
+        // node's own state stays in registers beside the network's
+        mlp_v8b a[4];
+#pragma unroll
+        for (int m = 0; m < 2; m++) {
+            const mlp_v16f z = mfma_32k8(w.w1[m][lw], xop);
+            a[2 * m] = gelu_oct(z, 0);
+            a[2 * m + 1] = gelu_oct(z, 8);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        float part = 0.f;
+#pragma unroll
+        for (int m = 0; m < 2; m++) {
+            mlp_v16f acc = w.b2[m][hw];
+#pragma unroll
+            for (int s = 0; s < 4; s++) acc = mfma_32k16(w.w2[m][s][lw], a[s], acc);
+            __builtin_amdgcn_sched_barrier(0);
+            const uint4 *w3 = (const uint4 *)w.w3[hw][m];
+            part = dot8(gelu_oct(acc, 0), w3[0], part);
+            part = dot8(gelu_oct(acc, 8), w3[1], part);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // both lane halves hold partial sums of sample 32t + c
+        const unsigned pb = __builtin_bit_cast(unsigned, part);
+        const auto sw = __builtin_amdgcn_permlane32_swap(pb, pb, false, false);
+        part = __builtin_bit_cast(float, (unsigned)sw[0]) + __builtin_bit_cast(float, (unsigned)sw[1]);
+        if (h == t) y = part;
     }
     wave_sync();
     return y + w.sc.b3;
