@@ -158,8 +158,9 @@ void choose_shape(lbmc_ctx *c)
     c->dx = make_fastdiv(c->L[0]);
     c->dy = make_fastdiv(c->L[1]);
     // the closure kernel stages its weights once per block and loops over
-    // 256-node chunks: a few blocks per CU
-    int cap = 2048;
+    // 256-node chunks; 8192 blocks (8 per resident slot at 4 blocks per CU)
+    // measured 2 % faster than 2048 or 1536 at 256^3, the same as 4096
+    int cap = 8192;
     if (const char *env = getenv("LBMC_CLOSURE_GRID"))
         if (atoi(env) > 0) cap = atoi(env);
     const int chunks = (int)((c->N + kBlock - 1) / kBlock);

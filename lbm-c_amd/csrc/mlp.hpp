@@ -21,7 +21,8 @@
 // of the result (2^-9 relative) that follows.
 //
 // Training's wave mapping (64 lanes, lane l = 16 g + r): a wave takes 64
-// samples as four tiles of 16 (mlp_wave_eval describes the evaluation's).  Each layer is computed transposed, H = W X, with
+// samples as four tiles of 16 (mlp_wave_eval describes the evaluation's).
+// Each layer is computed transposed, H = W X, with
 // weights as the A operand and samples as the columns of B, so the MFMA
 // result layout of one layer (lane holds rows 4g..4g+3 of column r) is the
 // B-operand layout of the next: no data moves between lanes inside the
