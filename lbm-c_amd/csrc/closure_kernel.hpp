@@ -121,8 +121,9 @@ __global__ __launch_bounds__(kBlock, 4) void closure_step_kernel(const StepArgs<
 template <class L, class St, bool GUO, bool WIDE>
 __global__ __launch_bounds__(kBlock) void closure_reuse_kernel(const StepArgs<float> a)
 {
-    const int n = (int)blockIdx.x * kBlock + (int)threadIdx.x;
-    if (n >= a.nnodes) return;
+    const unsigned nu = row_chunk(blockIdx.x, a.rlog) * (unsigned)kBlock + threadIdx.x;
+    if (nu >= (unsigned)a.nnodes) return;
+    const int n = (int)nu;
     int dd[3][3];
     slot_offsets<L>(a, n, dd);
     float v[L::Q];

@@ -42,6 +42,7 @@ struct lbmc_ctx {
     bool offsets64 = false;  // a direction array spans 4 GiB or more: only 64-bit offsets reach it
                              //   (key.wide may also be set below that, by choice)
     long long hs = 0;        // half-interleaved 16-bit layout: N / 2 (StepArgs::hs), else 0
+    unsigned rlog = 0;       // row_chunk block order of the narrow step kernels (StepArgs::rlog)
     double F[3] = {0, 0, 0};
     int ncls = 1;
     uint16_t cls_host[256] = {};
@@ -117,6 +118,7 @@ inline void fill_args(const lbmc_ctx *c, StepArgs<C> &a, long long t, int store)
     a.ly = c->dy.l;
     a.wide_rows = (c->key.wide && c->L[0] % 64 == 0) ? 1 : 0;
     a.hs = (unsigned)c->hs;
+    a.rlog = c->rlog;
     a.tau = (C)c->desc.tau;
     a.c_smag = (C)c->desc.c_smag;
     for (int i = 0; i < 15; i++) a.tau_mrt[i] = (C)c->desc.tau_mrt[i];

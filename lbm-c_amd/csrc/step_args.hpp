@@ -63,6 +63,7 @@ struct StepArgs {
     int lx, ly;              //   (q = (umulhi(n, m) + n) >> l, n < 2^31)
     int wide_rows;           // 64-bit-offset kernels with Lx % 64 == 0: every wave lies in
                              //   one x-row, so each direction's slot row is wave-uniform
+    unsigned rlog;           // blocks take 2^rlog consecutive 256-node chunks per XCD (row_chunk), 0: plain order
     unsigned hs;             // 16-bit storage in the half-interleaved layout: hs = nnodes / 2
                              //   (the halves of the last axis); slot k < hs is element 2k and
                              //   slot k + hs element 2k + 1 of the direction array, so the two

@@ -23,6 +23,7 @@
 
 #include "fast_bgk.hpp"
 #include "lattice.hpp"
+#include "node_index.hpp"
 #include "node_ops.hpp"
 #include "step_args.hpp"
 #include "storage.hpp"
@@ -458,9 +459,10 @@ __device__ __forceinline__ void slot_offsets(const StepArgs<C> &a, int n, int (&
 template <class L, class St, int FAM, bool GUO, bool SMAG, bool MFIX, bool WIDE>
 __global__ __launch_bounds__(kBlock) void step_kernel(const StepArgs<typename St::C> a)
 {
-    // grid = ceil(nnodes / 256) <= 2^23 blocks, so n <= 2^31 - 1
-    const int n = (int)blockIdx.x * kBlock + (int)threadIdx.x;
-    if (n >= a.nnodes) return;
+    // grid <= 2^23 + 2^5 blocks (row_grid): the node index is exact in unsigned
+    const unsigned nu = row_chunk(blockIdx.x, a.rlog) * (unsigned)kBlock + threadIdx.x;
+    if (nu >= (unsigned)a.nnodes) return;
+    const int n = (int)nu;
     int dd[3][3];
     slot_offsets<L>(a, n, dd);
     // WIDE = false (direction arrays below 4 GiB): a uniform base per
@@ -542,8 +544,8 @@ __global__ __launch_bounds__(kBlock) void step_half_kernel(const StepArgs<float>
     using S = typename St::S;
     static_assert(sizeof(S) == 2, "pairs of 16-bit slots");
     constexpr int Q = L::Q;
-    // grid = ceil(hs / 256) blocks: p < 2^30
-    const unsigned p = blockIdx.x * (unsigned)kBlock + threadIdx.x;
+    // grid = row_grid(ceil(hs / 256)) blocks: p < 2^31
+    const unsigned p = row_chunk(blockIdx.x, a.rlog) * (unsigned)kBlock + threadIdx.x;
     const unsigned hs = a.hs;
     if (p >= hs) return;
     const int n0 = (int)p, n1 = (int)(p + hs);

@@ -19,7 +19,7 @@ static hipError_t launch_one(const StepArgs<typename St::C> &a, int grid, hipStr
     if constexpr (sizeof(typename St::S) == 2) {
         // half-interleaved layout: two nodes per lane on every call
         if (pairs && !wide && a.hs != 0) {
-            const int hgrid = (int)((a.hs + kBlock - 1) / kBlock);
+            const int hgrid = (int)row_grid((a.hs + kBlock - 1) / kBlock, a.rlog);
             hipLaunchKernelGGL((step_half_kernel<L, St, FAM, GUO, SMAG, MFIX>), dim3(hgrid), dim3(kBlock), lds, s,
                                a);
             return hipGetLastError();

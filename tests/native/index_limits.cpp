@@ -4,10 +4,11 @@
 // the tiles near the end of the index range, every live index must lie in
 // [first node of its chunk, N), and the last node must be covered.
 //
-// The closure and batch tiles use lbmc::tile_count / lbmc::tile_node (the
-// header the kernels include).  The one-node and two-node step kernels index
-// with plain int arithmetic (their grids end exactly at the last node); it
-// is restated here so that a signed overflow in it would trap.
+// The closure and batch tiles use lbmc::tile_count / lbmc::tile_node, the
+// one-node step kernels lbmc::row_chunk / lbmc::row_grid (the header the
+// kernels include).  The two-node step kernel indexes with plain int
+// arithmetic (its grid ends exactly at the last node); it is restated here
+// so that a signed overflow in it would trap.
 //
 // Build: g++ -std=c++17 -fsanitize=undefined,address -fno-sanitize-recover=all
 // With -DSIGNED_TILES the closure loop is replayed with the int arithmetic it
@@ -16,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <initializer_list>
+#include <vector>
 
 #include "../../lbm-c_amd/csrc/node_index.hpp"
 
@@ -90,19 +92,51 @@ void batch_tiles(int N, int chunk_nodes)
     check(last, "batch covers the last node", N - 1, N);
 }
 
-// step_kernel: grid = ceil(N / 256), n = b * 256 + lane
-void step_tiles(int N)
+// step_kernel, step_half_kernel, closure_reuse_kernel: grid =
+// row_grid(ceil(N / 256), rlog), n = row_chunk(b, rlog) * 256 + lane in
+// unsigned arithmetic (rlog > 0 only where N is a multiple of 256 << rlog)
+void step_tiles(int N, unsigned rlog)
 {
-    const int grid = (int)(((long long)N + kBlock - 1) / kBlock);
+    const unsigned chunks = lbmc::tile_count(N, kBlock);
+    const unsigned grid = lbmc::row_grid(chunks, rlog);
+    check(grid >= chunks && grid - chunks < (8u << rlog), "step grid", grid, N);
     bool last = false;
-    for (int b = grid - 3 < 0 ? 0 : grid - 3; b < grid; b++)
-        for (int lane = 0; lane < kBlock; lane++) {
-            const int n = b * kBlock + lane;
-            if (n >= N) continue;
+    const unsigned tail = 16u << rlog;
+    for (unsigned b = grid > tail ? grid - tail : 0; b < grid; b++) {
+        const unsigned c = lbmc::row_chunk(b, rlog);
+        for (unsigned lane = 0; lane < (unsigned)kBlock; lane++) {
+            const unsigned nu = c * (unsigned)kBlock + lane;
+            if (nu >= (unsigned)N) continue;
+            const int n = (int)nu;
             check(n >= 0, "step node", n, N);
             if (n == N - 1) last = true;
         }
+    }
     check(last, "step covers the last node", N - 1, N);
+}
+
+// the row order over a whole small lattice: every node exactly once, and
+// the 2^rlog chunks of a group on one XCD (blocks are dealt b mod 8)
+void row_order_cover(int N, unsigned rlog)
+{
+    const unsigned chunks = lbmc::tile_count(N, kBlock);
+    const unsigned grid = lbmc::row_grid(chunks, rlog);
+    std::vector<unsigned char> seen((size_t)N, 0);
+    for (unsigned b = 0; b < grid; b++) {
+        const unsigned c = lbmc::row_chunk(b, rlog);
+        if (c < chunks) check((c >> rlog) % 8u == b % 8u, "row group on one XCD", c, N);
+        for (unsigned lane = 0; lane < (unsigned)kBlock; lane++) {
+            const unsigned nu = c * (unsigned)kBlock + lane;
+            if (nu >= (unsigned)N) continue;
+            check(seen[nu] == 0, "node taken twice", nu, N);
+            seen[nu] = 1;
+        }
+    }
+    for (int n = 0; n < N; n++)
+        if (!seen[(size_t)n]) {
+            check(false, "node not taken", n, N);
+            break;
+        }
 }
 
 // step_pair_kernel (N even): grid = ceil(N / 2 / 256), n = 2 (b * 256 + lane)
@@ -129,8 +163,16 @@ int main()
     for (int N : sizes) {
         for (unsigned grid : {1u, 7u, 2048u}) closure_tiles(N, grid);
         for (int cn : {256, 512, 768, 1024, 2048, 16384}) batch_tiles(N, cn);
-        step_tiles(N);
+        step_tiles(N, 0);
         if (N % 2 == 0) pair_tiles(N);
+    }
+    // row order: lattices of 512- and 1024-node rows near the limit and small
+    for (unsigned rlog : {1u, 2u}) {
+        const int row = kBlock << rlog;
+        for (int N : {limit / row * row, (limit / row - 7) * row, (1 << 30), 30 * row, 15 * row, row}) {
+            step_tiles(N, rlog);
+            if (N <= (1 << 20)) row_order_cover(N, rlog);
+        }
     }
     std::printf("%s\n", failures ? "index limits: FAILED" : "index limits: ok");
     return failures ? 1 : 0;
