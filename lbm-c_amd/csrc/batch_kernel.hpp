@@ -72,7 +72,8 @@ __global__ __launch_bounds__(kBlock) void batch_step_kernel(const BatchMember<ty
     using C = typename St::C;
     constexpr bool CL = std::is_same<C, float>::value;
     for (int c = (int)blockIdx.x; c < nchunks; c += (int)gridDim.x) {
-        const int2 ch = chunks[c];  // (member, first node)
+        const int2 ch = chunks[c];  // (member, first node); member -1: padding of the XCD order
+        if (ch.x < 0) continue;
         const BatchMember<C> &m = members[ch.x];
         const StepArgs<C> &a = m.a;
         const int variant = m.variant;
@@ -123,6 +124,7 @@ __global__ __launch_bounds__(kBlock, 4) void batch_eval_kernel(const BatchMember
     mlp_v8b *xw = xb + (threadIdx.x & ~63u);
     for (int c = (int)blockIdx.x; c < nchunks; c += (int)gridDim.x) {
         const int2 ch = chunks[c];
+        if (ch.x < 0) continue;
         const BatchMember<float> &m = members[ch.x];
         const StepArgs<float> &a = m.a;
         const int variant = m.variant;
