@@ -225,7 +225,9 @@ def main():
         # arithmetic, bit for bit
         for st in (lbmc.FP16S, lbmc.BF16S, lbmc.FP64):
             v, vtwin = steady(lambda: make_lattice(lbmc, L, st, fields))
-            vsteps = max(20, args.steps)
+            # 100 calls (~20 ms at 16 bits, ~75 ms at FP64): a steady-state
+            # window rather than a few milliseconds of the clock's swings
+            vsteps = max(100, args.steps)
             v.time_steps(0, args.warmup, 0)
             vms = v.time_steps(args.warmup, vsteps, 0)
             vtwin.close()
