@@ -156,12 +156,12 @@ int default_blocks_per_cu(const lbmc_ctx *c)
 void choose_shape(lbmc_ctx *c)
 {
     c->grid = (int)((c->N + kBlock - 1) / kBlock);
-    // narrow kernels on rows of R = 2^k >= 2 chunks: min(R, 4) consecutive
-    // chunks per XCD (row_chunk); groups of 4 measured best on 4096^2 D2Q9
-    // (2, 8 and whole 16-chunk rows slower), whole rows on 512^3 D3Q15 (+5 %)
+    // rows of R = 2^k >= 2 chunks: min(R, 4) consecutive chunks per XCD
+    // (row_chunk); groups of 4 measured best on 4096^2 D2Q9 (2, 8 and whole
+    // 16-chunk rows slower), whole rows on 512^3 D3Q15 (+5 %)
     c->rlog = 0;
     const char *ro = getenv("LBMC_ROW_ORDER");
-    if (!c->key.wide && c->L[0] % (2 * kBlock) == 0 && !(ro && atoi(ro) == 0)) {
+    if (c->L[0] % (2 * kBlock) == 0 && !(ro && atoi(ro) == 0)) {
         const int R = c->L[0] / kBlock;
         if ((R & (R - 1)) == 0)
             while ((1 << c->rlog) < R && c->rlog < 2) c->rlog++;

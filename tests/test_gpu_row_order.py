@@ -1,6 +1,7 @@
-"""Block order of the one-block-per-chunk step kernels on lattices whose
-x-rows hold R = 2 or 4 chunks of 256 nodes: the library gives each XCD
-min(R, 4) consecutive chunks (node_index.hpp row_chunk), so that the cache
+"""Block order of the one-block-per-chunk step kernels (32- and 64-bit
+offsets) on lattices whose x-rows hold R = 2 or 4 chunks of 256 nodes: the
+library gives each XCD min(R, 4) consecutive chunks (node_index.hpp
+row_chunk), so that the cache
 line a rotated row splits at a chunk boundary stays in one L2.  The order
 changes which block updates a node, not the update: the FP64 step stays
 bit-identical to the oracle on such lattices, and every storage, the
@@ -57,11 +58,14 @@ def _build(case, storage, env):
                 os.environ[k] = v
 
 
+@pytest.mark.parametrize("wide", [False, True], ids=["narrow", "wide"])
 @pytest.mark.parametrize("case", ROW_CASES, ids=[c["name"] for c in ROW_CASES])
-def test_row_order_fp64_matches_oracle(case):
-    print(f"[row order fp64] {case['name']}", flush=True)
+def test_row_order_fp64_matches_oracle(case, wide):
+    """Narrow kernels, and the 64-bit-offset row kernels forced on the same
+    lattices (LBMC_FORCE_WIDE=1), in the row order."""
+    print(f"[row order fp64] {case['name']} wide={wide}", flush=True)
     o = cases.build(pyoracle.Oracle, case)
-    g = _build(case, lbmc.FP64, {})
+    g = _build(case, lbmc.FP64, {"LBMC_FORCE_WIDE": "1"} if wide else {})
     for t0 in (0, 2 ** 31 + 5):
         o.run(t0, case["steps"], 3)
         g.run(t0, case["steps"], 3)
@@ -78,14 +82,16 @@ def test_row_order_matches_plain_order_bitwise(storage):
         for t0 in (0, 1_000_003):
             print(f"[row order] {case['name']} storage={storage} t0={t0}", flush=True)
             out = []
-            for env in ({}, {"LBMC_ROW_ORDER": "0"}):
+            for env in ({}, {"LBMC_ROW_ORDER": "0"}, {"LBMC_FORCE_WIDE": "1"},
+                        {"LBMC_FORCE_WIDE": "1", "LBMC_ROW_ORDER": "0"}):
                 lat = _build(case, storage, env)
                 lat.run(t0, case["steps"], 4)
                 out.append((lat.get_f(), lat.get_macros()))
                 lat.close()
-            (fa, (ra, ua)), (fb, (rb, ub)) = out
-            assert np.array_equal(fa, fb), case["name"]
-            assert np.array_equal(ra, rb) and np.array_equal(ua, ub), case["name"]
+            fa, (ra, ua) = out[0]
+            for k, (fb, (rb, ub)) in enumerate(out[1:]):
+                assert np.array_equal(fa, fb), (case["name"], k)
+                assert np.array_equal(ra, rb) and np.array_equal(ua, ub), (case["name"], k)
 
 
 def test_row_order_closure_reuse_bitwise():
