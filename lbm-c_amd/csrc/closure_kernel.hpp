@@ -102,8 +102,13 @@ __global__ __launch_bounds__(kBlock, 4) void closure_step_kernel(const StepArgs<
     }
     __syncthreads();
     mlp_v8b *xw = xb + (threadIdx.x & ~63u);
+    // chunks in row_chunk order (the step kernels' XCD-aware order); the
+    // grid is a multiple of 8 << rlog, so position p runs on XCD p mod 8
     const unsigned nchunks = tile_count(a.nnodes, kBlock);
-    for (unsigned b = blockIdx.x; b < nchunks; b += gridDim.x) {
+    const unsigned npos = row_grid(nchunks, a.rlog);
+    for (unsigned p = blockIdx.x; p < npos; p += gridDim.x) {
+        const unsigned b = row_chunk(p, a.rlog);
+        if (b >= nchunks) continue;  // block-uniform
         const unsigned n0 = tile_node(0u, b, kBlock, threadIdx.x);
         const bool live = n0 < (unsigned)a.nnodes;
         const int n = live ? (int)n0 : a.nnodes - 1;

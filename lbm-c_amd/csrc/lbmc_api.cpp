@@ -176,7 +176,7 @@ void choose_shape(lbmc_ctx *c)
     if (const char *env = getenv("LBMC_CLOSURE_GRID"))
         if (atoi(env) > 0) cap = atoi(env);
     const int chunks = (int)((c->N + kBlock - 1) / kBlock);
-    c->closure_grid = std::min(chunks, cap);
+    c->closure_grid = std::min((int)row_grid((unsigned)chunks, c->rlog), cap);
     c->reuse_grid = (int)row_grid((unsigned)chunks, c->rlog);
 }
 

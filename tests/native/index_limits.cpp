@@ -34,23 +34,32 @@ void check(bool ok, const char *what, long long n, long long N)
     }
 }
 
-// closure_step_kernel: blocks stride over 256-node chunks, lanes past the
+// closure_step_kernel: blocks stride over the positions of the 256-node
+// chunks in row_chunk order (rlog 0: the chunks themselves), lanes past the
 // last node evaluate a copy of node N - 1 and write nothing
-void closure_tiles(int N, unsigned grid)
+void closure_tiles(int N, unsigned grid, unsigned rlog = 0)
 {
 #ifdef SIGNED_TILES
     const int nchunks = (N + kBlock - 1) / kBlock;
+    const int npos = nchunks;
 #else
     const unsigned nchunks = lbmc::tile_count(N, kBlock);
+    const unsigned npos = lbmc::row_grid(nchunks, rlog);
 #endif
     check((long long)nchunks * kBlock >= N && ((long long)nchunks - 1) * kBlock < N, "closure chunk count",
           nchunks, N);
     bool last = false;
     for (unsigned blk = 0; blk < grid; blk++) {
-        // the block's last few chunks only: the ones that approach N
-        unsigned b = blk;
-        if (nchunks > 4 * grid) b += (nchunks / grid - 3) * grid;
-        for (; b < (unsigned)nchunks; b += grid)
+        // the block's last few positions only: the ones that approach N
+        unsigned p = blk;
+        if ((unsigned)npos > 4 * grid) p += ((unsigned)npos / grid - 3) * grid;
+        for (; p < (unsigned)npos; p += grid) {
+#ifdef SIGNED_TILES
+            const unsigned b = p;
+#else
+            const unsigned b = lbmc::row_chunk(p, rlog);
+            if (b >= nchunks) continue;
+#endif
             for (unsigned lane = 0; lane < (unsigned)kBlock; lane++) {
 #ifdef SIGNED_TILES
                 const int n0 = (int)b * kBlock + (int)lane;
@@ -64,6 +73,7 @@ void closure_tiles(int N, unsigned grid)
                 check(n >= 0 && n < N, "closure node", n, N);
                 if (live && n == N - 1) last = true;
             }
+        }
     }
     check(last, "closure covers the last node", N - 1, N);
 }
@@ -171,6 +181,7 @@ int main()
         const int row = kBlock << rlog;
         for (int N : {limit / row * row, (limit / row - 7) * row, (1 << 30), 30 * row, 15 * row, row}) {
             step_tiles(N, rlog);
+            for (unsigned grid : {8u, 2048u, 8192u}) closure_tiles(N, grid, rlog);
             if (N <= (1 << 20)) row_order_cover(N, rlog);
         }
     }
