@@ -290,6 +290,11 @@ lbmc_status lbmc_batch_autotune(lbmc_batch *b, long long t0, int steps_per_trial
  * blocks_per_cu: cap on the blocks resident per CU, 3..16, or 0 for none
  * (ignored when members use the learned closure). */
 lbmc_status lbmc_batch_set_shape(lbmc_batch *b, int grid, int chunk_nodes, int blocks_per_cu);
+/* Shape report: nodes over all members, grid, chunks (entries of the chunk
+ * list: every member's chunks of chunk_nodes nodes and, for chunks of 256
+ * or 512 nodes, the padding of the XCD-aware order, a few entries that the
+ * kernels skip), chunk_nodes and the residency cap.  Any pointer may be
+ * NULL. */
 lbmc_status lbmc_batch_info(const lbmc_batch *b, long long *nodes, int *grid, int *chunks, int *chunk_nodes,
                             int *blocks_per_cu);
 
