@@ -182,8 +182,8 @@ lbmc_status lbmc_copy_bandwidth(int device, size_t nbytes, int reps, float *ms);
  *
  * Replaces the reference's Smagorinsky hook turbulent_viscosity
  * (LBM-C/collision.cu:445-463, called by bgk_collision at :60-62) with a
- * 7 -> 64 -> 64 -> 1 GELU perceptron evaluated on the matrix cores (bf16
- * operands, FP32 accumulation).  Inputs per node: Pi_ab / rho for ab = xx,
+ * 7 -> 64 -> 64 -> 1 GELU perceptron on the matrix cores (evaluated with
+ * fp16 operands, trained with bf16 operands; FP32 accumulation).  Inputs per node: Pi_ab / rho for ab = xx,
  * yy, zz, xy, yz, xz with Pi = sum_i e_i e_i (f_i - feq_i), then rho; each
  * normalised as (x - mu) * isd.  Output y gives tau' = out_a * y + out_b;
  * outside [tau_lo, tau_hi] the node falls back to the Smagorinsky tau
@@ -205,7 +205,8 @@ typedef struct lbmc_mlp_scaling {
 } lbmc_mlp_scaling;
 
 /* Trainer / model on one device: FP32 master weights, Adam state, and the
- * bf16 operand image the kernels read.  sc may be null (identity). */
+ * operand images the kernels read (bf16 for training, fp16 for evaluation).
+ * sc may be null (identity). */
 lbmc_status lbmc_mlp_create(int device, const float *params, const lbmc_mlp_scaling *sc, lbmc_mlp **out);
 void lbmc_mlp_destroy(lbmc_mlp *m);
 const char *lbmc_mlp_last_error(const lbmc_mlp *m);

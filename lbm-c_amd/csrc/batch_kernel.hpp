@@ -114,14 +114,14 @@ __global__ __launch_bounds__(kBlock, 4) void batch_eval_kernel(const BatchMember
 {
     if (*any_eval == 0) return;
     __shared__ MlpEval w;
-    __shared__ mlp_v8b xb[kBlock];
+    __shared__ mlp_h8 xb[kBlock];
     {
         const uint4 *src = (const uint4 *)&((const MlpImage *)mlp)->ev;
         uint4 *dst = (uint4 *)&w;
         for (int k = (int)threadIdx.x; k < (int)(sizeof(MlpEval) / 16); k += kBlock) dst[k] = src[k];
     }
     __syncthreads();
-    mlp_v8b *xw = xb + (threadIdx.x & ~63u);
+    mlp_h8 *xw = xb + (threadIdx.x & ~63u);
     for (int c = (int)blockIdx.x; c < nchunks; c += (int)gridDim.x) {
         const int2 ch = chunks[c];
         if (ch.x < 0) continue;

@@ -7,8 +7,8 @@
 //      the backward pass and its partial gradient (per-wave record);
 //   2. mlp_update_kernel: one thread per parameter sums the records in wave
 //      order (deterministic) and applies Adam to the FP32 master weights;
-//   3. mlp_image_kernel: rebuilds the bf16 operand image from the master
-//      weights.
+//   3. mlp_image_kernel: rebuilds the operand images (bf16 for training,
+//      fp16 for evaluation) from the master weights.
 // The loss is the mean squared error over the minibatch.
 #include <hip/hip_runtime.h>
 
@@ -69,15 +69,15 @@ __global__ __launch_bounds__(64) void mlp_image_kernel(const float *p, MlpScalar
             v[j] = __builtin_bit_cast(float, (unsigned)bf16_bits(w3[16 * h + 4 * g + j]) << 16);
         img->w3d[h][l] = v;
     } else if (b < 38) {
-        // evaluation image (32x32 products)
+        // evaluation image (32x32 products, fp16 operands)
         const int m = (b - 30) >> 2, s = (b - 30) & 3, h = l >> 5, c = l & 31;
-        mlp_v8b v;
-        for (int j = 0; j < 8; j++) v[j] = bf16_of(W2[(32 * m + c) * 64 + mlp_hid32(s, 8 * h + j)]);
+        mlp_h8 v;
+        for (int j = 0; j < 8; j++) v[j] = (_Float16)W2[(32 * m + c) * 64 + mlp_hid32(s, 8 * h + j)];
         img->ev.w2[m][s][l] = v;
     } else if (b < 40) {
         const int m = b - 38, h = l >> 5, c = l & 31;
-        mlp_v4s v;
-        for (int j = 0; j < 4; j++) v[j] = (short)bf16_bits(W1[(32 * m + c) * 8 + 4 * h + j]);
+        mlp_v4h v;
+        for (int j = 0; j < 4; j++) v[j] = (_Float16)W1[(32 * m + c) * 8 + 4 * h + j];
         img->ev.w1[m][l] = v;
     } else if (b == 40) {
         const int m = l >> 5, h = (l >> 4) & 1, e = l & 15;
@@ -85,9 +85,8 @@ __global__ __launch_bounds__(64) void mlp_image_kernel(const float *p, MlpScalar
     } else if (b == 41) {
         if (l < 32) {
             const int h = l >> 4, m = (l >> 3) & 1, q = l & 7;
-            const unsigned lo = bf16_bits(w3[32 * m + mlp_row32(h, 2 * q)]);
-            const unsigned hi = bf16_bits(w3[32 * m + mlp_row32(h, 2 * q + 1)]);
-            img->ev.w3[h][m][q] = lo | (hi << 16);
+            const mlp_h2 v = {(_Float16)w3[32 * m + mlp_row32(h, 2 * q)], (_Float16)w3[32 * m + mlp_row32(h, 2 * q + 1)]};
+            img->ev.w3[h][m][q] = __builtin_bit_cast(unsigned, v);
         }
     } else if (l == 0) {
         MlpScalars s = sc;
@@ -102,7 +101,7 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(const MlpImage *img, c
                                                           float *y)
 {
     __shared__ MlpEval w;
-    __shared__ mlp_v8b xb[256];
+    __shared__ mlp_h8 xb[256];
     {
         const uint4 *src = (const uint4 *)&img->ev;
         uint4 *dst = (uint4 *)&w;
@@ -113,7 +112,7 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(const MlpImage *img, c
     const bool valid = i < S;
     float f[7];
     for (int k = 0; k < 7; k++) f[k] = valid ? feat[i * 7 + k] : 0.f;
-    const float out = mlp_wave_eval(w, xb + (threadIdx.x & ~63u), mlp_input(w.sc, f));
+    const float out = mlp_wave_eval(w, xb + (threadIdx.x & ~63u), mlp_input_h(w.sc, f));
     if (valid) y[i] = out;
 }
 

@@ -4,21 +4,25 @@
 // called from bgk_collision at :60-62): tau' = out_a * y + out_b, and the
 // Smagorinsky tau is used wherever tau' leaves the band [lo, hi].
 //
-// Arithmetic: bf16 operands, FP32 accumulation; biases and derivatives in
-// FP32.  Training (mlp_tile_forward and the backward pass) uses 16x16 tiles
-// (MFMA 16x16x16 bf16 for the first layer, 16x16x32 bf16 for the others);
-// evaluation in the closure steps (mlp_wave_eval) uses 32x32 tiles (32x32x8
-// and 32x32x16 bf16, the output layer as bf16 dot products).
-// GELU(z) = z Phi(z), Phi from an odd degree-9 polynomial (least-squares fit
-// of the normal CDF on [-3.5, 3.5], |error| <= 8.6e-4 in exact arithmetic),
-// clamped to [0, 1].  The polynomial runs two activations at a time in packed
-// FP16 (gelu_pair): x = fp16(z), t = x*x, four fused multiply-adds, the
-// [0, 1] clamp folded into the last one, then z * Phi in FP32 and the bf16
-// rounding the next product needs -- 10 vector instructions per pair against
-// 17 for the same polynomial in FP32.  Over every finite fp16 x this Phi is
-// within 1.8e-3 of the normal CDF and exactly 0 / 1 beyond -3.5 / 3.5, so
-// the input needs no clamp; |GELU error| <= 5.2e-3, below the bf16 half-ulp
-// of the result (2^-9 relative) that follows.
+// Arithmetic: training (mlp_tile_forward and the backward pass) takes bf16
+// operands on 16x16 tiles (MFMA 16x16x16 bf16 for the first layer, 16x16x32
+// bf16 for the others); evaluation in the closure steps (mlp_wave_eval)
+// takes fp16 operands on 32x32 tiles (32x32x8 and 32x32x16 f16, the output
+// layer as fp16 dot products).  Accumulation, biases and derivatives are
+// FP32.  GELU(z) = z Phi(z), Phi from an odd degree-9 polynomial
+// (least-squares fit of the normal CDF on [-3.5, 3.5], |error| <= 8.6e-4 in
+// exact arithmetic), clamped to [0, 1].  The polynomial runs in packed FP16
+// (gelu_pair, gelu_oct): x = fp16(z), t = x*x, four fused multiply-adds, the
+// [0, 1] clamp folded into the last one.  Training then forms z * Phi in
+// FP32 and rounds it to bf16 (10 vector instructions per pair of
+// activations against 17 for the same polynomial in FP32); evaluation forms
+// x * Phi in packed fp16, the next product's operand as it stands (8 per
+// pair).  Over every finite fp16 x this Phi is within 1.8e-3 of the normal
+// CDF and exactly 0 / 1 beyond -3.5 / 3.5, so the input needs no clamp;
+// |GELU error| <= 5.2e-3 (bf16 result) and the half-ulp of the fp16 result
+// is 2^-11 relative.  An fp16 pre-activation beyond 65504 makes x infinite:
+// the evaluation's GELU is then inf or NaN and tau' leaves its band (the
+// Smagorinsky tau is used).
 //
 // Training's wave mapping (64 lanes, lane l = 16 g + r): a wave takes 64
 // samples as four tiles of 16 (mlp_wave_eval describes the evaluation's).
@@ -51,6 +55,8 @@ constexpr int MLP_PSTRIDE = 4740;
 typedef short mlp_v4s __attribute__((ext_vector_type(4)));
 typedef __bf16 mlp_v8b __attribute__((ext_vector_type(8)));
 typedef float mlp_v4f __attribute__((ext_vector_type(4)));
+typedef _Float16 mlp_v4h __attribute__((ext_vector_type(4)));
+typedef _Float16 mlp_h8 __attribute__((ext_vector_type(8)));
 
 // hidden unit held at K position 8g + j of the 32-wide chunk c
 __host__ __device__ constexpr int mlp_hid(int c, int g, int j) { return 16 * (2 * c + (j >> 2)) + 4 * g + (j & 3); }
@@ -84,13 +90,13 @@ __host__ __device__ constexpr int mlp_hid32(int s, int k)
     return 16 * s + 8 * ((k & 7) >> 2) + 4 * (k >> 3) + (k & 3);
 }
 
-// Operand-ordered weights of the closure's evaluation (mlp_wave_eval):
+// Operand-ordered fp16 weights of the closure's evaluation (mlp_wave_eval):
 // 32x32 products, lane l = 32h + c.
 struct MlpEval {
-    mlp_v8b w2[2][4][64];  // A of layer 2, row tile m, K-step s: W2[32m + c][hid32(s, 8h + j)]
-    mlp_v4s w1[2][64];     // A of layer 1, row tile m: W1[32m + c][4h + j]
+    mlp_h8 w2[2][4][64];   // A of layer 2, row tile m, K-step s: W2[32m + c][hid32(s, 8h + j)]
+    mlp_v4h w1[2][64];     // A of layer 1, row tile m: W1[32m + c][4h + j]
     mlp_v16f b2[2][2];     // layer-2 accumulator start, row tile m, lane half h: b2[32m + row32(h, e)]
-    unsigned w3[2][2][8];  // bf16 pairs (low, high) of w3[32m + row32(h, 2q)], w3[32m + row32(h, 2q + 1)], [h][m][q]
+    unsigned w3[2][2][8];  // fp16 pairs (low, high) of w3[32m + row32(h, 2q)], w3[32m + row32(h, 2q + 1)], [h][m][q]
     MlpScalars sc;
 };
 
@@ -252,17 +258,15 @@ __device__ __forceinline__ void mlp_tile_forward(const MlpInfer &w, const mlp_v8
     o.y = yv[0];
 }
 
-__device__ __forceinline__ mlp_v16f mfma_32k8(mlp_v4s a, mlp_v4s b)
+__device__ __forceinline__ mlp_v16f mfma_32k8(mlp_v4h a, mlp_v4h b)
 {
-    return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, mlp_v16f{}, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_32x32x8f16(a, b, mlp_v16f{}, 0, 0, 0);
 }
 
-__device__ __forceinline__ mlp_v16f mfma_32k16(mlp_v8b a, mlp_v8b b, mlp_v16f c)
+__device__ __forceinline__ mlp_v16f mfma_32k16(mlp_h8 a, mlp_h8 b, mlp_v16f c)
 {
-    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
 }
-
-typedef _Float16 mlp_h8 __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ mlp_h8 mlp_h8_splat(float c)
 {
@@ -270,9 +274,10 @@ __device__ __forceinline__ mlp_h8 mlp_h8_splat(float c)
     return mlp_h8{h, h, h, h, h, h, h, h};
 }
 
-// gelu_pair on z[e0 .. e0 + 7] (four pairs whose packed operations issue side
-// by side), as the bf16 operand of the next product, element j = z[e0 + j]
-__device__ __forceinline__ mlp_v8b gelu_oct(const mlp_v16f &z, int e0)
+// GELU of z[e0 .. e0 + 7] as the fp16 operand of the next product: the Phi
+// of gelu_pair (four pairs whose packed operations issue side by side), then
+// x * Phi rounded to fp16 once, x = fp16(z); element j = z[e0 + j]
+__device__ __forceinline__ mlp_h8 gelu_oct(const mlp_v16f &z, int e0)
 {
     mlp_h8 x;
 #pragma unroll
@@ -284,20 +289,28 @@ __device__ __forceinline__ mlp_v8b gelu_oct(const mlp_v16f &z, int e0)
     p = __builtin_elementwise_fma(t, p, mlp_h8_splat(GELU_C1));
     mlp_h8 s = __builtin_elementwise_fma(x, p, mlp_h8_splat(0.5f));
     s = __builtin_elementwise_min(__builtin_elementwise_max(s, mlp_h8_splat(0.f)), mlp_h8_splat(1.f));
-    mlp_v8b r;
-#pragma unroll
-    for (int j = 0; j < 8; j++) r[j] = (__bf16)__builtin_fmaf(z[e0 + j], (float)s[j], 0.f);
-    return r;
+    return x * s;
 }
 
-// sum over the four bf16 pairs of a and b, added to c in FP32
-__device__ __forceinline__ float dot8(mlp_v8b a, uint4 b, float c)
+// sum over the four fp16 pairs of a and b, added to c in FP32
+__device__ __forceinline__ float dot8(mlp_h8 a, uint4 b, float c)
 {
     const uint4 ua = __builtin_bit_cast(uint4, a);
-    c = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(mlp_b2, ua.x), __builtin_bit_cast(mlp_b2, b.x), c, false);
-    c = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(mlp_b2, ua.y), __builtin_bit_cast(mlp_b2, b.y), c, false);
-    c = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(mlp_b2, ua.z), __builtin_bit_cast(mlp_b2, b.z), c, false);
-    return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(mlp_b2, ua.w), __builtin_bit_cast(mlp_b2, b.w), c, false);
+    c = __builtin_amdgcn_fdot2(__builtin_bit_cast(mlp_h2, ua.x), __builtin_bit_cast(mlp_h2, b.x), c, false);
+    c = __builtin_amdgcn_fdot2(__builtin_bit_cast(mlp_h2, ua.y), __builtin_bit_cast(mlp_h2, b.y), c, false);
+    c = __builtin_amdgcn_fdot2(__builtin_bit_cast(mlp_h2, ua.z), __builtin_bit_cast(mlp_h2, b.z), c, false);
+    return __builtin_amdgcn_fdot2(__builtin_bit_cast(mlp_h2, ua.w), __builtin_bit_cast(mlp_h2, b.w), c, false);
+}
+
+// a sample's raw features -> normalised fp16 input row of the evaluation
+// (constant-1 bias input last)
+__device__ __forceinline__ mlp_h8 mlp_input_h(const MlpScalars &s, const float (&f)[7])
+{
+    mlp_h8 r;
+#pragma unroll
+    for (int k = 0; k < 7; k++) r[k] = (_Float16)((f[k] - s.mu[k]) * s.isd[k]);
+    r[7] = (_Float16)1.f;
+    return r;
 }
 
 // Network output for the lane's own sample (input row xin), from the
@@ -309,12 +322,12 @@ __device__ __forceinline__ float dot8(mlp_v8b a, uint4 b, float c)
 // layer 1 is two 32x32x8 products (K = the 7 inputs and the bias input),
 // layer 2 two row tiles of four 32x32x16 products whose K order mlp_hid32 is
 // the order in which the lanes hold layer 1's GELUs, and the output layer a
-// dot product of each lane's 32 layer-2 GELUs with w3 (bf16 pairs, FP32 sum)
+// dot product of each lane's 32 layer-2 GELUs with w3 (fp16 pairs, FP32 sum)
 // plus the other lane half's.  Against the 16x16 tiles of mlp_tile_forward
 // (training) this skips the zero K rows of layer 1 and the 15 unused rows of
 // the output layer's product, a quarter of the matrix-core work, and reads
-// half the weight operands per sample; only the FP32 summation order differs.
-__device__ __forceinline__ float mlp_wave_eval(const MlpEval &w, mlp_v8b *xb, mlp_v8b xin)
+// half the weight operands per sample.
+__device__ __forceinline__ float mlp_wave_eval(const MlpEval &w, mlp_h8 *xb, mlp_h8 xin)
 {
     const int l = (int)(threadIdx.x & 63), h = l >> 5, c = l & 31;
     xb[l] = xin;
@@ -327,10 +340,10 @@ __device__ __forceinline__ float mlp_wave_eval(const MlpEval &w, mlp_v8b *xb, ml
 #pragma unroll 1
     for (int t = 0; t < 2; t++) {
         // layer-1 B operand: inputs 4h..4h+3 of sample 32t + c
-        const mlp_v4s xop = ((const mlp_v4s *)&xb[32 * t + c])[h];
+        const mlp_v4h xop = ((const mlp_v4h *)&xb[32 * t + c])[h];
         // one 32-row tile at a time, phases fenced from the scheduler: the
         // node's own state stays in registers beside the network's
-        mlp_v8b a[4];
+        mlp_h8 a[4];
 #pragma unroll
         for (int m = 0; m < 2; m++) {
             const mlp_v16f z = mfma_32k8(w.w1[m][lw], xop);

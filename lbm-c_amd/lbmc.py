@@ -366,7 +366,8 @@ def _f32(a, shape_tail=()):
 
 class Mlp:
     """The closure network on one device: FP32 master weights, Adam state and
-    the bf16 operand image the step and training kernels read.
+    the operand images the kernels read (fp16 for the closure steps and
+    forward, bf16 for training).
 
     params: LBMC_MLP_NPARAM floats (w1[64][8] with the bias in column 7,
     w2[64][64], b2[64], w3[64], b3).  Inputs are normalised as

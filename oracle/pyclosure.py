@@ -8,7 +8,7 @@ optional uniform Guo force, the reference's call semantics
 (LBM-C/solver.cu:10-136): call 0 collides in place, every later call first
 pulls f_i(x) <- f_i(x - e_i) and then collides; solids bounce back
 (LBM-C/collision.cu:423-443).  At fluid nodes tau comes from the network
-(oracle/pymlp.py, bf16 quantisation points emulated) as out_a * y + out_b
+(oracle/pymlp.py, the evaluation's fp16 quantisation points emulated) as out_a * y + out_b
 + shift (shift: the lattice's offset, 0 by default; evaluated on calls t % every == 0 and the first call, the recorded tau
 reused in between), or from the Smagorinsky formula (LBM-C/collision.cu:445-463, restated in
 closed form: tau' = (tau + sqrt(tau^2 + 36 C^2 (sum|Pi_aa| + 2 sum_{a<b}
@@ -103,7 +103,7 @@ class ClosureOracle:
         rho, u, feq, pi, x = self.features(f)
         if self.every <= 1 or self.record is None or self.calls % self.every == 0:
             xn = ((x.astype(np.float32) - self.mu) * self.isd).astype(np.float64)
-            y, _ = pymlp.forward(self.p, xn, quant=self.quant)
+            y, _ = pymlp.forward(self.p, xn, quant="eval" if self.quant else False)
             t = self.out_a * y + self.out_b + self.shift
             self.tau_net = t
             bad = ~((t >= self.lo) & (t <= self.hi))

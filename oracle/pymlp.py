@@ -14,20 +14,25 @@ normal CDF on [-3.5, 3.5]) added to 1/2 and clamped to [0, 1].
     multiply-add clamped to [0, 1], no input clamp; GELU = fp32(z * Phi).
     |error| <= 1.8e-3 in Phi over every finite fp16 input, exactly 0 / 1
     beyond -3.5 / 3.5.
+  * gelu16h: the evaluation's GELU (mlp.hpp gelu_oct): the same Phi, then
+    fp16(x * Phi) with x = fp16(z).
 tests/test_closure_math.py checks all of these against erf.
 
 Parameters (one dict of float64 arrays):
     w1 (64, 8)  hidden x (7 normalised inputs + a constant-1 bias column)
     w2 (64, 64), b2 (64,), w3 (64,), b3 ()
 
-`quant=True` evaluates the activations and their derivatives as the device
-does (gelu16, dgelu16) and rounds, as the device does, the inputs, the
-weights, both hidden activations and both hidden-layer error signals to
-bfloat16 (round to nearest even, through float32) before they enter a
-product; sums stay in float64.  The device accumulates in float32, so
-quantised results agree with the device to float32 summation error (and the
-rare fp16 / bf16 rounding that lands on the other side of a tie because of
-it).
+`quant=True` is the training kernels' arithmetic (lbm-c_amd/csrc/mlp.hip
+mlp_train_kernel): activations and their derivatives as the device
+evaluates them (gelu16, dgelu16), and the inputs, the weights, both hidden
+activations and both hidden-layer error signals rounded to bfloat16 (round
+to nearest even, through float32) before they enter a product.
+`quant="eval"` is the evaluation's (mlp.hpp mlp_wave_eval: the closure
+steps and lbmc_mlp_forward): inputs and weights rounded to fp16 through
+float32, activations gelu16h (already fp16), layer-2 bias in float32.  Sums
+stay in float64.  The device accumulates in float32, so quantised results
+agree with the device to float32 summation error (and the rare fp16 / bf16
+rounding that lands on the other side of a tie because of it).
 """
 from __future__ import annotations
 
@@ -106,6 +111,20 @@ def gelu16(z):
     return (z32 * phi16(z32)).astype(np.float32).astype(np.float64)
 
 
+def fp16(a):
+    """Round to fp16 (nearest even) through float32, as the device converts
+    FP32 values; returned as float64."""
+    return _fp16(np.asarray(a, dtype=np.float32).astype(np.float64))
+
+
+def gelu16h(z):
+    """The evaluation's GELU: x = fp16(z), the device's Phi of x, and the
+    product x * Phi rounded to fp16 once (exact in float64 before it)."""
+    x = fp16(z)
+    with np.errstate(invalid="ignore"):
+        return _fp16(x * np.clip(_phi16_raw(x), 0.0, 1.0))
+
+
 def dgelu16(z):
     """The device's GELU derivative: Phi(x) + z Phi'(x), x = fp16(z), with
     Phi and its clamp test from the fp16 evaluation and Phi' exact."""
@@ -143,7 +162,17 @@ def unflatten(v):
 
 
 def forward(p, x, quant=False):
-    """x: (S, 7) normalised features -> y (S,), cache."""
+    """x: (S, 7) normalised features -> y (S,), cache.  quant: False (exact),
+    True (training arithmetic) or "eval" (evaluation arithmetic)."""
+    if quant == "eval":
+        X = np.concatenate([x, np.ones((x.shape[0], 1))], axis=1)
+        Xq, W1, W2, w3 = fp16(X), fp16(p["w1"]), fp16(p["w2"]), fp16(p["w3"])
+        z1 = Xq @ W1.T
+        a1 = gelu16h(z1)
+        z2 = a1 @ W2.T + np.asarray(p["b2"], dtype=np.float32).astype(np.float64)
+        a2 = gelu16h(z2)
+        y = a2 @ w3 + p["b3"]
+        return y, dict(Xq=Xq, W2=W2, w3=w3, z1=z1, a1=a1, z2=z2, a2=a2)
     q = bf16 if quant else (lambda a: np.asarray(a, dtype=np.float64))
     act = gelu16 if quant else gelu
     X = np.concatenate([x, np.ones((x.shape[0], 1))], axis=1)

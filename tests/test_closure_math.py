@@ -39,6 +39,28 @@ def test_gelu16_matches_normal_cdf_on_every_fp16_input():
     assert np.max(np.abs(pymlp.phi16(-x) - (1 - s))) <= 2.0 ** -11
 
 
+def test_gelu16h_matches_normal_cdf_on_every_fp16_input():
+    """The evaluation's GELU: an fp16 value within the fp16 Phi's error plus
+    half an fp16 ulp of x Phi_true, exact tails, and the fp16 rounding of
+    x Phi (not of z Phi in FP32, not bf16)."""
+    x = _all_fp16()
+    phi_true = 0.5 * (1 + erf(x / np.sqrt(2)))
+    g = pymlp.gelu16h(x)
+    assert not np.any(np.isnan(g))
+    assert np.array_equal(g, pymlp._fp16(g))
+    small = np.abs(x) <= 8
+    ref = x[small] * phi_true[small]
+    bound = 1.8e-3 * np.abs(x[small]) + 2.0 ** -11 * np.abs(ref) + 2.0 ** -25
+    assert np.all(np.abs(g[small] - ref) <= bound)
+    assert np.array_equal(g[x > 3.5], x[x > 3.5]) and np.all(g[x < -3.5] == 0.0)
+    # fp16 results agree with the training form's FP32 product to half an fp16 ulp
+    g32 = pymlp.gelu16(x)
+    assert np.all(np.abs(g - g32) <= 2.0 ** -11 * np.abs(g32) + 2.0 ** -25)
+    # a value rounded twice on its way: FP32 pre-activation, then fp16
+    assert pymlp.fp16(1.0 + 2 ** -11 + 2 ** -30) == 1.0     # 2^-30 lost in float32: a tie, to even
+    assert pymlp._fp16(1.0 + 2 ** -11 + 2 ** -30) == 1.0 + 2 ** -10
+
+
 def test_fp16_rounding():
     f = pymlp._fp16
     assert f(1.0 + 2 ** -11) == 1.0                  # tie -> even
@@ -110,3 +132,14 @@ def test_quantised_forward_close_to_exact():
     y0, _ = pymlp.forward(p, x)
     y1, _ = pymlp.forward(p, x, quant=True)
     assert np.max(np.abs(y1 - y0)) < 0.05 * np.max(np.abs(y0))
+    # the evaluation's fp16 operands carry 3 more mantissa bits than bf16:
+    # its rounding error is the smaller, the GELU approximation common to both
+    y2, _ = pymlp.forward(p, x, quant="eval")
+    assert np.max(np.abs(y2 - y0)) < 0.05 * np.max(np.abs(y0))
+    # the same network with the device's GELU and no operand rounding
+    X = np.concatenate([x, np.ones((x.shape[0], 1))], axis=1)
+    a1 = pymlp.gelu16(X @ p["w1"].T)
+    a2 = pymlp.gelu16(a1 @ p["w2"].T + p["b2"])
+    yr = a2 @ p["w3"] + p["b3"]
+    e1, e2 = np.abs(y1 - yr), np.abs(y2 - yr)
+    assert np.median(e2) < 0.25 * np.median(e1), (np.median(e1), np.median(e2))

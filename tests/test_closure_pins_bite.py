@@ -64,3 +64,27 @@ def test_bf16_pin_catches_truncation(monkeypatch):
 
     monkeypatch.setattr(pymlp, "bf16", truncate)
     _fails(pins.test_bf16_rounding)
+
+
+def test_gelu16h_pin_catches_unrounded_product(monkeypatch):
+    def unrounded(z):
+        x = pymlp.fp16(z)
+        return x * np.clip(pymlp._phi16_raw(x), 0.0, 1.0)
+
+    monkeypatch.setattr(pymlp, "gelu16h", unrounded)
+    _fails(pins.test_gelu16h_matches_normal_cdf_on_every_fp16_input)
+
+
+def test_gelu16h_pin_catches_bf16_product(monkeypatch):
+    def to_bf16(z):
+        x = pymlp.fp16(z)
+        return pymlp.bf16(x * np.clip(pymlp._phi16_raw(x), 0.0, 1.0))
+
+    monkeypatch.setattr(pymlp, "gelu16h", to_bf16)
+    _fails(pins.test_gelu16h_matches_normal_cdf_on_every_fp16_input)
+
+
+def test_eval_forward_pin_catches_bf16_operands(monkeypatch):
+    # the evaluation form with the training form's bf16 operand rounding
+    monkeypatch.setattr(pymlp, "fp16", pymlp.bf16)
+    _fails(pins.test_quantised_forward_close_to_exact)

@@ -1,8 +1,9 @@
 """Learned relaxation-time closure on the GPU against the FP64 oracles
 (oracle/pymlp.py for the network, oracle/pyclosure.py for the step):
-forward values, loss and gradient of the training kernels (bf16 rounding
-points emulated in the oracle, FP32 accumulation on the device), the Adam
-step, training progress and determinism, and the fused closure step."""
+forward values of the evaluation (fp16 rounding points emulated in the
+oracle), loss and gradient of the training kernels (bf16 rounding points
+emulated), FP32 accumulation on the device; the Adam step, training
+progress and determinism, and the fused closure step."""
 import numpy as np
 import pytest
 
@@ -35,16 +36,16 @@ def test_forward_matches_oracle():
     m = lbmc.Mlp(pymlp.flatten(p), mu=mu, isd=isd)
     y = m.forward(feat).astype(np.float64)
     xn = ((feat - mu) * isd).astype(np.float64)
-    yq, _ = pymlp.forward(p, xn, quant=True)
+    yq, _ = pymlp.forward(p, xn, quant="eval")
     y0, _ = pymlp.forward(p, xn, quant=False)
     scale = np.max(np.abs(yq))
     err = np.abs(y - yq)
     print(f"[closure] forward: median {np.median(err) / scale:.2e}, max {np.max(err) / scale:.2e} of max|y|",
           flush=True)
-    # FP32 accumulation vs FP64 with the same fp16 activation arithmetic and
-    # bf16 operands: typical differences are FP32 summation error (~1e-8 of
-    # max|y|); an fp16 or bf16 rounding that lands on the other side of a tie
-    # moves one activation by one ulp, which bounds the worst sample
+    # FP32 accumulation vs FP64 with the same fp16 operands and activation
+    # arithmetic: typical differences are FP32 summation error (~1e-8 of
+    # max|y|); an fp16 rounding that lands on the other side of a tie moves
+    # one activation by one ulp, which bounds the worst sample
     assert np.median(err) < 1e-6 * scale
     assert np.max(err) < 1e-3 * scale
     # and the quantised network stays close to the exact one
