@@ -538,12 +538,20 @@ __global__ __launch_bounds__(kBlock) void step_pair_kernel(const StepArgs<float>
 // the FP32 step's pattern.  Each node takes step_kernel's update and
 // stochastic-rounding words: bit-identical results.  Direction arrays below
 // 4 GiB only.
+//
+// Which half is n0's depends on the last axis alone (the other axes' slot
+// offsets stay within n0's plane or row, which lies in the lower half):
+// always the low half for directions with no last-axis component, else one
+// flag per sign of that component.  The flags fold into the last-axis word
+// offsets and into a byte selector (v_perm_b32) that swaps the halves where
+// n0 holds the high one, on the loaded and on the stored word alike.
 template <class L, class St, int FAM, bool GUO, bool SMAG, bool MFIX>
 __global__ __launch_bounds__(kBlock) void step_half_kernel(const StepArgs<float> a)
 {
     using S = typename St::S;
     static_assert(sizeof(S) == 2, "pairs of 16-bit slots");
     constexpr int Q = L::Q;
+    constexpr int A = L::D - 1;  // the split axis
     // grid = row_grid(ceil(hs / 256)) blocks: p < 2^31
     const unsigned p = row_chunk(blockIdx.x, a.rlog) * (unsigned)kBlock + threadIdx.x;
     const unsigned hs = a.hs;
@@ -551,24 +559,30 @@ __global__ __launch_bounds__(kBlock) void step_half_kernel(const StepArgs<float>
     const int n0 = (int)p, n1 = (int)(p + hs);
     int dd[3][3];
     slot_offsets<L>(a, n0, dd);
+    // word offsets along the split axis, [e + 1] for component e, and the
+    // half selectors of the two signs
+    constexpr unsigned keep = 0x03020100u, swap = 0x01000302u;
+    const bool lowp = (unsigned)(n0 + dd[A][2]) < hs, lowm = (unsigned)(n0 + dd[A][0]) < hs;
+    int dw[3];
+    dw[1] = 0;
+    dw[2] = lowp ? dd[A][2] : dd[A][2] - (int)hs;
+    dw[0] = lowm ? dd[A][0] : dd[A][0] - (int)hs;
+    const unsigned selp = lowp ? keep : swap, selm = lowm ? keep : swap;
     float v0[Q], v1[Q];
-    unsigned lowmask = 0;  // bit i: node n0 holds the low half of direction i's word
     constexpr bool PRIO = St::gather_prio && L::D == 3;
     if constexpr (PRIO) __builtin_amdgcn_s_setprio(1);
     sfor<0, Q>([&](auto ii) {
         constexpr int i = decltype(ii)::value;
-        const unsigned k = (unsigned)(n0 + dd[0][L::ex[i] + 1] + dd[1][L::ey[i] + 1] + dd[2][L::ez[i] + 1]);
-        const bool low = k < hs;
-        const unsigned w = low ? k : k - hs;
+        constexpr int e = A == 2 ? L::ez[i] : L::ey[i];
+        const int other = A == 2 ? dd[0][L::ex[i] + 1] + dd[1][L::ey[i] + 1] : dd[0][L::ex[i] + 1];
+        const unsigned w = (unsigned)(n0 + other + dw[e + 1]);
         const char *base = (const char *)((const S *)a.f + (size_t)i * a.npad);
-        const unsigned word = ldg((const unsigned *)(base + w * 4u));
-        // both halves converted in place, then assigned to the nodes
+        unsigned word = ldg((const unsigned *)(base + w * 4u));
+        if constexpr (e != 0) word = __builtin_amdgcn_perm(word, word, e > 0 ? selp : selm);
         typedef S s2 __attribute__((ext_vector_type(2)));
         const s2 wv = __builtin_bit_cast(s2, word);
-        const float a0 = St::load(wv.x), a1 = St::load(wv.y);
-        v0[i] = low ? a0 : a1;
-        v1[i] = low ? a1 : a0;
-        lowmask |= low ? (1u << i) : 0u;
+        v0[i] = St::load(wv.x);
+        v1[i] = St::load(wv.y);
     });
     if constexpr (PRIO) {
         __builtin_amdgcn_sched_barrier(0);
@@ -580,16 +594,17 @@ __global__ __launch_bounds__(kBlock) void step_half_kernel(const StepArgs<float>
     unsigned r0 = sr_seed(n0, ts), r1 = sr_seed(n1, ts);
     sfor<0, Q>([&](auto ii) {
         constexpr int i = decltype(ii)::value;
-        const unsigned k = (unsigned)(n0 + dd[0][L::ex[i] + 1] + dd[1][L::ey[i] + 1] + dd[2][L::ez[i] + 1]);
-        const bool low = (lowmask >> i) & 1u;
-        const unsigned w = low ? k : k - hs;
+        constexpr int e = A == 2 ? L::ez[i] : L::ey[i];
+        const int other = A == 2 ? dd[0][L::ex[i] + 1] + dd[1][L::ey[i] + 1] : dd[0][L::ex[i] + 1];
+        const unsigned w = (unsigned)(n0 + other + dw[e + 1]);
         // node n0's value in the low half, then the halves swapped where n0
         // holds the high one
-        const unsigned word = St::store_sr_pair(v0[i], r0, v1[i], r1);
+        unsigned word = St::store_sr_pair(v0[i], r0, v1[i], r1);
         r0 = sr_next(r0);
         r1 = sr_next(r1);
+        if constexpr (e != 0) word = __builtin_amdgcn_perm(word, word, e > 0 ? selp : selm);
         char *base = (char *)((S *)a.f + (size_t)i * a.npad);
-        stg((unsigned *)(base + w * 4u), low ? word : __builtin_amdgcn_alignbit(word, word, 16));
+        stg((unsigned *)(base + w * 4u), word);
     });
 }
 
