@@ -325,6 +325,18 @@ __device__ __forceinline__ void wide_row_slots(const StepArgs<typename St::C> &a
     });
 }
 
+// A lane's 32-bit byte offset, opaque to the compiler: its widening to 64
+// bits then happens beside the access that uses it, so a store in another
+// basic block than the gather that computed the same offset still takes a
+// scalar base and the 32-bit offset (otherwise the compiler carries the
+// widened offset over and adds 64-bit per-lane addresses: two vector
+// instructions and two registers per direction).  No instruction.
+__device__ __forceinline__ unsigned lane_offset(unsigned off)
+{
+    asm volatile("" : "+v"(off));
+    return off;
+}
+
 // Element of slot k in a direction array: k, or in the half-interleaved
 // 16-bit layout (hs = N / 2 > 0) 2k for k < hs and 2(k - hs) + 1 above, so
 // that slots k and k + hs (nodes half the last axis apart) share a word.
@@ -405,7 +417,7 @@ __device__ __forceinline__ void scatter_node(const StepArgs<typename St::C> &a, 
         } else {
             char *base = (char *)(fb + (size_t)i * a.npad);
             const unsigned e = sizeof(S) == 2 ? slot_elem((unsigned)k, a.hs) : (unsigned)k;
-            stg((S *)(base + e * (unsigned)sizeof(S)), s);
+            stg((S *)(base + lane_offset(e * (unsigned)sizeof(S))), s);
         }
     });
 }
@@ -604,7 +616,7 @@ __global__ __launch_bounds__(kBlock) void step_half_kernel(const StepArgs<float>
         r1 = sr_next(r1);
         if constexpr (e != 0) word = __builtin_amdgcn_perm(word, word, e > 0 ? selp : selm);
         char *base = (char *)((S *)a.f + (size_t)i * a.npad);
-        stg((unsigned *)(base + w * 4u), word);
+        stg((unsigned *)(base + lane_offset(w * 4u)), word);
     });
 }
 
