@@ -14,14 +14,19 @@
 //                         members that evaluate the network are skipped;
 //   batch_eval_kernel     (closure members only) those chunks, with the
 //                         network; returns at once when no member evaluates;
-//   batch_advance_kernel  one block moves every member's arguments from
-//                         call t to t + 1 (shift counters, slot offsets, the
-//                         macro-store flag, the closure phase) and sets the
-//                         device flag of the evaluation kernel, so no kernel
-//                         takes t as an argument.
-// Because nothing in the launches depends on t, a fixed run of calls is
-// captured once into a HIP graph and replayed for any t.  The launch shape
-// of the step kernel is chosen by lbmc_batch_autotune (timed candidates).
+// The kernels of call t + j read the members' arguments of that call from
+// entry j of a window of kBatchWindow calls in device memory (and the
+// evaluation kernel its flag any_eval[j]).  At the end of a window of calls
+//   batch_advance_kernel  one block moves every member's state from call t
+//                         to t + calls (shift counters, slot offsets, the
+//                         macro-store flag, the closure phase), and
+//   batch_window_kernel   one block per entry lays out the next window and
+//                         its evaluation flags,
+// so no kernel takes t as an argument and the state moves on once per
+// window.  Because nothing in the launches depends on t, a window of calls
+// is captured once into a HIP graph and replayed for any t.  The launch
+// shape of the step kernel is chosen by lbmc_batch_autotune (timed
+// candidates).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -45,13 +50,14 @@ struct lbmc_batch {
     std::vector<lbmc_ctx *> members;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    void *dmembers = nullptr;  // BatchMember<C>[count]
+    void *dmembers = nullptr;  // BatchMember<C>[count]: the state of call t_next
+    void *dwindow = nullptr;   // BatchMember<C>[kBatchWindow][count]: calls t_next + j
     int2 *chunks = nullptr;
     int nchunks = 0;
     int2 *cchunks = nullptr;  // the closure members' chunks
     int ncchunks = 0;
     int eval_grid = 0;        // blocks of the evaluation kernel
-    int *any_eval = nullptr;  // device flag: some closure member evaluates this call
+    int *any_eval = nullptr;  // device flags [kBatchWindow]: some closure member evaluates call t_next + j
     const void *mlp = nullptr;  // closure weights shared by the closure members (a member's image)
     long long nodes = 0;
     int grid = 1;
@@ -105,7 +111,6 @@ template <class C>
 lbmc_status upload_members(lbmc_batch *b, long long t0, int store_every)
 {
     std::vector<BatchMember<C>> hm((size_t)b->count);
-    int any_eval = 0;  // the flag batch_advance_kernel keeps for later calls
     for (int k = 0; k < b->count; k++) {
         const lbmc_ctx *c = b->members[(size_t)k];
         if (c->nsf || c->offsets64 || !c->fields_set)
@@ -126,35 +131,50 @@ lbmc_status upload_members(lbmc_batch *b, long long t0, int store_every)
         m.closure_every = c->key.closure ? c->closure_every : 1;
         m.closure_phase = (int)(t0 % m.closure_every);
         if (m.closure_every > 1) m.a.tmode = (!c->tcache_valid || m.closure_phase == 0) ? 1 : 2;
-        if (m.variant >= 8 && m.a.tmode != 2) any_eval = 1;
     }
     B_HIP(b, hipMemcpyAsync(b->dmembers, hm.data(), sizeof(BatchMember<C>) * hm.size(), hipMemcpyHostToDevice,
                             b->stream));
-    B_HIP(b, hipMemcpyAsync(b->any_eval, &any_eval, sizeof(int), hipMemcpyHostToDevice, b->stream));
+    hipLaunchKernelGGL(batch_window_kernel<C>, dim3(kBatchWindow), dim3(256), 0, b->stream,
+                       (const BatchMember<C> *)b->dmembers, b->count, (BatchMember<C> *)b->dwindow, b->any_eval);
+    B_HIP(b, hipGetLastError());
     B_HIP(b, hipStreamSynchronize(b->stream));
     b->t_next = t0;
     return LBMC_OK;
 }
 
-lbmc_status launch_one_step(lbmc_batch *b)
+// call t_next + j (j < kBatchWindow) from window entry j
+lbmc_status launch_one_step(lbmc_batch *b, int j)
 {
     hipError_t e;
+    const size_t msz = b->cdouble ? sizeof(BatchMember<double>) : sizeof(BatchMember<float>);
+    const void *members = (const char *)b->dwindow + (size_t)j * (size_t)b->count * msz;
     const BatchShape sh{b->grid, b->sub, b->lds};
-    const BatchEval ev{b->cchunks, b->mlp ? b->ncchunks : 0, b->eval_grid, b->mlp, b->any_eval};
+    const BatchEval ev{b->cchunks, b->mlp ? b->ncchunks : 0, b->eval_grid, b->mlp, b->any_eval + j};
     switch (b->storage) {
-    case LBMC_FP64: e = launch_batch_fp64(b->lattice, b->dmembers, b->chunks, b->nchunks, b->grid, b->sub, b->lds, b->stream); break;
-    case LBMC_FP32: e = launch_batch_st<StFP32>(b->lattice, b->dmembers, b->chunks, b->nchunks, sh, ev, b->stream); break;
-    case LBMC_FP16S: e = launch_batch_st<StFP16S>(b->lattice, b->dmembers, b->chunks, b->nchunks, sh, ev, b->stream); break;
-    default: e = launch_batch_st<StBF16S>(b->lattice, b->dmembers, b->chunks, b->nchunks, sh, ev, b->stream); break;
+    case LBMC_FP64: e = launch_batch_fp64(b->lattice, members, b->chunks, b->nchunks, b->grid, b->sub, b->lds, b->stream); break;
+    case LBMC_FP32: e = launch_batch_st<StFP32>(b->lattice, members, b->chunks, b->nchunks, sh, ev, b->stream); break;
+    case LBMC_FP16S: e = launch_batch_st<StFP16S>(b->lattice, members, b->chunks, b->nchunks, sh, ev, b->stream); break;
+    default: e = launch_batch_st<StBF16S>(b->lattice, members, b->chunks, b->nchunks, sh, ev, b->stream); break;
     }
     if (e != hipSuccess) return bfail(b, LBMC_ERR_DEVICE, "batch step launch: %s", hipGetErrorString(e));
-    if (b->cdouble)
-        hipLaunchKernelGGL(batch_advance_kernel<double>, dim3(1), dim3(256), 0, b->stream,
-                           (BatchMember<double> *)b->dmembers, b->count, b->any_eval);
-    else
-        hipLaunchKernelGGL(batch_advance_kernel<float>, dim3(1), dim3(256), 0, b->stream,
-                           (BatchMember<float> *)b->dmembers, b->count, b->any_eval);
-    e = hipGetLastError();
+    return LBMC_OK;
+}
+
+// the state and the window move on by `calls` (<= kBatchWindow) calls
+template <class C>
+void launch_advance(lbmc_batch *b, int calls)
+{
+    hipLaunchKernelGGL(batch_advance_kernel<C>, dim3(1), dim3(256), 0, b->stream, (BatchMember<C> *)b->dmembers,
+                       b->count, calls);
+    hipLaunchKernelGGL(batch_window_kernel<C>, dim3(kBatchWindow), dim3(256), 0, b->stream,
+                       (const BatchMember<C> *)b->dmembers, b->count, (BatchMember<C> *)b->dwindow, b->any_eval);
+}
+
+lbmc_status advance(lbmc_batch *b, int calls)
+{
+    if (b->cdouble) launch_advance<double>(b, calls);
+    else launch_advance<float>(b, calls);
+    const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return bfail(b, LBMC_ERR_DEVICE, "batch advance launch: %s", hipGetErrorString(e));
     return LBMC_OK;
 }
@@ -168,7 +188,7 @@ void drop_batch_graph(lbmc_batch *b)
     b->graph_steps = 0;
 }
 
-constexpr int kBatchGraphSteps = 32;
+constexpr int kBatchGraphSteps = kBatchWindow;
 
 lbmc_status upload(lbmc_batch *b, long long t0, int store_every)
 {
@@ -183,7 +203,8 @@ lbmc_status batch_steps(lbmc_batch *b, long long t0, int nsteps)
         if (!b->graph_exec) {
             B_HIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
             lbmc_status st = LBMC_OK;
-            for (int k = 0; k < kBatchGraphSteps && st == LBMC_OK; k++) st = launch_one_step(b);
+            for (int k = 0; k < kBatchGraphSteps && st == LBMC_OK; k++) st = launch_one_step(b, k);
+            if (st == LBMC_OK) st = advance(b, kBatchGraphSteps);
             hipGraph_t g = nullptr;
             const hipError_t e = hipStreamEndCapture(b->stream, &g);
             if (st != LBMC_OK) {
@@ -200,10 +221,13 @@ lbmc_status batch_steps(lbmc_batch *b, long long t0, int nsteps)
             left -= b->graph_steps;
         }
     }
-    for (; left > 0; left--) {
-        lbmc_status st = launch_one_step(b);
+    // the rest of the run: calls of the current window, then one advance
+    for (int j = 0; j < left; j++) {
+        lbmc_status st = launch_one_step(b, j);
         if (st != LBMC_OK) return st;
     }
+    if (left > 0)
+        if (lbmc_status st = advance(b, left)) return st;
     b->t_next = t0 + nsteps;
     return LBMC_OK;
 }
@@ -345,7 +369,8 @@ lbmc_status lbmc_batch_create(lbmc_ctx *const *members, int count, lbmc_batch **
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
     const size_t msz = b->cdouble ? sizeof(BatchMember<double>) : sizeof(BatchMember<float>);
     if (e == hipSuccess) e = hipMalloc(&b->dmembers, msz * (size_t)count);
-    if (e == hipSuccess) e = hipMalloc((void **)&b->any_eval, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&b->dwindow, msz * (size_t)count * kBatchWindow);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->any_eval, sizeof(int) * kBatchWindow);
     if (e != hipSuccess) return bail(bfail(b, LBMC_ERR_DEVICE, "lbmc_batch_create: %s", hipGetErrorString(e)));
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, b->device) == hipSuccess && prop.multiProcessorCount > 0)
@@ -367,6 +392,7 @@ lbmc_status lbmc_batch_destroy(lbmc_batch *b)
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     drop_batch_graph(b);
     if (b->dmembers) (void)hipFree(b->dmembers);
+    if (b->dwindow) (void)hipFree(b->dwindow);
     if (b->chunks) (void)hipFree(b->chunks);
     if (b->cchunks) (void)hipFree(b->cchunks);
     if (b->any_eval) (void)hipFree(b->any_eval);

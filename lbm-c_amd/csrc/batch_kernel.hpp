@@ -65,6 +65,43 @@ __device__ __forceinline__ void batch_node(const StepArgs<typename St::C> &a, in
 
 __device__ __forceinline__ bool evaluates(int variant, int tmode) { return variant >= 8 && tmode != 2; }
 
+// Calls per window: the members' arguments of kBatchWindow consecutive calls
+// are laid out in device memory (batch_window_kernel), and the calls of a
+// window launch the step kernels on their own entry, so the members' state
+// moves on (batch_advance_kernel) once per window.
+constexpr int kBatchWindow = 32;
+
+// Arguments of call t + j from the member's state for call t: the shift
+// counters and slot offsets, the macro-store flag and the closure mode that
+// j single-call advances leave (j <= 2 kBatchWindow: the loops below run at
+// most that often).
+template <class C>
+__device__ __forceinline__ void call_args(const BatchMember<C> &m, int j, StepArgs<C> &a)
+{
+    long long stride = 1;
+    for (int d = 0; d < 3; d++) {
+        const int L = a.L[d];
+        int s = a.s[d] + j;
+        while (s >= L) s -= L;
+        a.s[d] = s;
+        a.dp[d] = (int)(-(long long)s * stride);
+        a.dpw[d] = (int)((long long)(L - s) * stride);
+        a.dm[d] = (int)((long long)s * stride);
+        a.dmw[d] = (int)((long long)(s - L) * stride);
+        stride *= L;
+    }
+    if (m.store_every > 0) {
+        int sc = m.store_count + j;
+        while (sc >= m.store_every) sc -= m.store_every;
+        a.store = sc == 0 ? 1 : 0;
+    }
+    if (m.closure_every > 1) {
+        int ph = m.closure_phase + j;
+        while (ph >= m.closure_every) ph -= m.closure_every;
+        a.tmode = ph == 0 ? 1 : 2;
+    }
+}
+
 template <class L, class St>
 __global__ __launch_bounds__(kBlock) void batch_step_kernel(const BatchMember<typename St::C> *__restrict__ members,
                                                             const int2 *__restrict__ chunks, int nchunks, int sub)
@@ -102,7 +139,7 @@ __global__ __launch_bounds__(kBlock) void batch_step_kernel(const BatchMember<ty
 }
 
 // Closure members' chunks (their own list) on a call where at least one of
-// them evaluates (*any_eval, written by batch_advance_kernel): the network
+// them evaluates (*any_eval, written by batch_window_kernel): the network
 // over whole waves, lanes past the member's last node evaluating a copy of
 // it and writing nothing.  The shared weights are staged into LDS once per
 // block.
@@ -151,43 +188,45 @@ __global__ __launch_bounds__(kBlock, 4) void batch_eval_kernel(const BatchMember
     }
 }
 
-// call t -> t + 1 for every member (one block, a thread per member), and
-// *any_eval for call t + 1: whether some closure member evaluates then
+// state: call t -> t + calls for every member (one block, a thread per
+// member; calls <= kBatchWindow)
 template <class C>
-__global__ __launch_bounds__(256) void batch_advance_kernel(BatchMember<C> *members, int count, int *any_eval)
+__global__ __launch_bounds__(256) void batch_advance_kernel(BatchMember<C> *state, int count, int calls)
 {
-    int ev = 0;
     for (int k = (int)threadIdx.x; k < count; k += (int)blockDim.x) {
-        BatchMember<C> m = members[k];
-        long long stride = 1;
-        for (int d = 0; d < 3; d++) {
-            const int L = m.a.L[d];
-            int s = m.a.s[d] + 1;
-            if (s == L) s = 0;
-            m.a.s[d] = s;
-            m.a.dp[d] = (int)(-(long long)s * stride);
-            m.a.dpw[d] = (int)((long long)(L - s) * stride);
-            m.a.dm[d] = (int)((long long)s * stride);
-            m.a.dmw[d] = (int)((long long)(s - L) * stride);
-            stride *= L;
-        }
+        BatchMember<C> m = state[k];
+        call_args(m, calls, m.a);
         if (m.store_every > 0) {
-            int sc = m.store_count + 1;
-            if (sc == m.store_every) sc = 0;
+            int sc = m.store_count + calls;
+            while (sc >= m.store_every) sc -= m.store_every;
             m.store_count = sc;
-            m.a.store = sc == 0 ? 1 : 0;
         }
         if (m.closure_every > 1) {
-            int ph = m.closure_phase + 1;
-            if (ph == m.closure_every) ph = 0;
+            int ph = m.closure_phase + calls;
+            while (ph >= m.closure_every) ph -= m.closure_every;
             m.closure_phase = ph;
-            m.a.tmode = ph == 0 ? 1 : 2;
         }
+        state[k] = m;
+    }
+}
+
+// window[q][k]: member k's arguments of call t + q (block q; the state
+// describes call t), and any_eval[q]: whether some closure member
+// evaluates the network on that call
+template <class C>
+__global__ __launch_bounds__(256) void batch_window_kernel(const BatchMember<C> *state, int count,
+                                                           BatchMember<C> *window, int *any_eval)
+{
+    const int q = (int)blockIdx.x;
+    int ev = 0;
+    for (int k = (int)threadIdx.x; k < count; k += (int)blockDim.x) {
+        BatchMember<C> m = state[k];
+        if (q != 0) call_args(m, q, m.a);
+        window[(size_t)q * (size_t)count + (size_t)k] = m;
         if (evaluates(m.variant, m.a.tmode)) ev = 1;
-        members[k] = m;
     }
     ev = __syncthreads_or(ev);
-    if (threadIdx.x == 0) *any_eval = ev;
+    if (threadIdx.x == 0) any_eval[q] = ev;
 }
 
 // Launch shape of the batch step: grid blocks, `sub` 256-node tiles per
