@@ -278,6 +278,16 @@ def main():
             closure_line[f"every_{every}"] = {
                 "MLUPS": round(N * csteps / (cms * 1e-3) / 1e6, 1), "ms_per_step": round(cms / csteps, 5),
                 "vs_smagorinsky_step": round((cms / csteps) / sms, 3)}
+        # the same network evaluated with fp16 operands (lbmc_mlp_set_eval_fp16)
+        model.set_eval_fp16(True)
+        c, ctwin = steady(lambda: closure_lattice(1))
+        c.time_steps(0, c0, 0)
+        cms = c.time_steps(c0, csteps, 0)
+        c.close()
+        ctwin.close()
+        closure_line["every_1_fp16_operands"] = {
+            "MLUPS": round(N * csteps / (cms * 1e-3) / 1e6, 1), "ms_per_step": round(cms / csteps, 5),
+            "vs_smagorinsky_step": round((cms / csteps) / sms, 3)}
         model.close()
         log(f"bench: closure {closure_line}")
 

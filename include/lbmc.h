@@ -182,8 +182,8 @@ lbmc_status lbmc_copy_bandwidth(int device, size_t nbytes, int reps, float *ms);
  *
  * Replaces the reference's Smagorinsky hook turbulent_viscosity
  * (LBM-C/collision.cu:445-463, called by bgk_collision at :60-62) with a
- * 7 -> 64 -> 64 -> 1 GELU perceptron on the matrix cores (evaluated with
- * fp16 operands, trained with bf16 operands; FP32 accumulation).  Inputs per node: Pi_ab / rho for ab = xx,
+ * 7 -> 64 -> 64 -> 1 GELU perceptron on the matrix cores (bf16 operands,
+ * FP32 accumulation; the evaluation optionally with fp16 operands).  Inputs per node: Pi_ab / rho for ab = xx,
  * yy, zz, xy, yz, xz with Pi = sum_i e_i e_i (f_i - feq_i), then rho; each
  * normalised as (x - mu) * isd.  Output y gives tau' = out_a * y + out_b;
  * outside [tau_lo, tau_hi] the node falls back to the Smagorinsky tau
@@ -205,7 +205,7 @@ typedef struct lbmc_mlp_scaling {
 } lbmc_mlp_scaling;
 
 /* Trainer / model on one device: FP32 master weights, Adam state, and the
- * operand images the kernels read (bf16 for training, fp16 for evaluation).
+ * operand images the kernels read (bf16, and an fp16 evaluation image).
  * sc may be null (identity). */
 lbmc_status lbmc_mlp_create(int device, const float *params, const lbmc_mlp_scaling *sc, lbmc_mlp **out);
 void lbmc_mlp_destroy(lbmc_mlp *m);
@@ -217,6 +217,15 @@ lbmc_status lbmc_mlp_set_scaling(lbmc_mlp *m, const lbmc_mlp_scaling *sc);
 /* Network outputs y[S] for raw features feat[S][7] (host arrays), by the
  * same device routine the closure step uses. */
 lbmc_status lbmc_mlp_forward(lbmc_mlp *m, const float *feat, long long S, float *y);
+
+/* Evaluation with fp16 operands (on != 0) instead of bf16 (the default) in
+ * lbmc_mlp_forward and in the closure steps of lattices that attach the
+ * model afterwards (lbmc_set_closure reads it; a batch takes it from its
+ * members).  Same tiles and FP32 accumulation; fp16 has 3 more mantissa bits
+ * and its activations cost 8 instead of 10 vector instructions per pair;
+ * pre-activations beyond 65504 send tau' out of its band (the Smagorinsky
+ * fallback).  No counterpart in the reference (the closure is an addition). */
+lbmc_status lbmc_mlp_set_eval_fp16(lbmc_mlp *m, int on);
 
 /* Mean squared error of y against target[S] and its gradient (one batch,
  * S <= 2^24), by the training kernels; grad may be null. */

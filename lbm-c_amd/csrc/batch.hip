@@ -72,6 +72,7 @@ struct lbmc_batch {
     int graph_steps = 0;
     const lbmc_mlp *closure_model = nullptr;
     double closure_lo = 0, closure_hi = 0;
+    int closure_fp16 = 0;  // the closure members' network with fp16 operands
     std::string err;
 };
 
@@ -116,8 +117,8 @@ lbmc_status upload_members(lbmc_batch *b, long long t0, int store_every)
         if (c->nsf || c->offsets64 || !c->fields_set)
             return bfail(b, LBMC_ERR_UNSUPPORTED, "batch member %d: sf nodes or state not supported", k);
         if (c->key.closure && (c->closure_model != b->closure_model || c->closure_lo != b->closure_lo ||
-                               c->closure_hi != b->closure_hi))
-            return bfail(b, LBMC_ERR_UNSUPPORTED, "batch member %d: closure members must share one model and band", k);
+                               c->closure_hi != b->closure_hi || c->key.closure_fp16 != b->closure_fp16))
+            return bfail(b, LBMC_ERR_UNSUPPORTED, "batch member %d: closure members must share one model, band and operand precision", k);
         BatchMember<C> &m = hm[(size_t)k];
         memset(&m, 0, sizeof(m));
         const int se = store_every > 0 ? store_every : 0;
@@ -149,7 +150,8 @@ lbmc_status launch_one_step(lbmc_batch *b, int j)
     const size_t msz = b->cdouble ? sizeof(BatchMember<double>) : sizeof(BatchMember<float>);
     const void *members = (const char *)b->dwindow + (size_t)j * (size_t)b->count * msz;
     const BatchShape sh{b->grid, b->sub, b->lds};
-    const BatchEval ev{b->cchunks, b->mlp ? b->ncchunks : 0, b->eval_grid, b->mlp, b->any_eval + j};
+    const BatchEval ev{b->cchunks, b->mlp ? b->ncchunks : 0, b->eval_grid, b->mlp, b->any_eval + j,
+                       b->closure_fp16 != 0};
     switch (b->storage) {
     case LBMC_FP64: e = launch_batch_fp64(b->lattice, members, b->chunks, b->nchunks, b->grid, b->sub, b->lds, b->stream); break;
     case LBMC_FP32: e = launch_batch_st<StFP32>(b->lattice, members, b->chunks, b->nchunks, sh, ev, b->stream); break;
@@ -319,6 +321,7 @@ lbmc_status lbmc_batch_create(lbmc_ctx *const *members, int count, lbmc_batch **
     long long nchunks = 0, nodes = 0;
     const lbmc_mlp *closure_model = nullptr;
     double closure_lo = 0, closure_hi = 0;
+    int closure_fp16 = 0;
     const void *closure_img = nullptr;
     for (int k = 0; k < count; k++) {
         const lbmc_ctx *c = members[k];
@@ -330,10 +333,12 @@ lbmc_status lbmc_batch_create(lbmc_ctx *const *members, int count, lbmc_batch **
         if (c->nsf) return bfail(nullptr, LBMC_ERR_UNSUPPORTED, "lbmc_batch_create: member %d has sf boundary nodes", k);
         if (c->key.closure) {
             if (closure_model && (c->closure_model != closure_model || c->closure_lo != closure_lo ||
-                                  c->closure_hi != closure_hi))
+                                  c->closure_hi != closure_hi || c->key.closure_fp16 != closure_fp16))
                 return bfail(nullptr, LBMC_ERR_UNSUPPORTED,
-                             "lbmc_batch_create: closure members must share one model and band (member %d)", k);
+                             "lbmc_batch_create: closure members must share one model, band and operand precision "
+                             "(member %d)", k);
             closure_model = c->closure_model;
+            closure_fp16 = c->key.closure_fp16;
             closure_lo = c->closure_lo;
             closure_hi = c->closure_hi;
             closure_img = c->mlp_img;
@@ -355,6 +360,7 @@ lbmc_status lbmc_batch_create(lbmc_ctx *const *members, int count, lbmc_batch **
     b->nchunks = (int)nchunks;
     b->nodes = nodes;
     b->closure_model = closure_model;
+    b->closure_fp16 = closure_fp16;
     b->closure_lo = closure_lo;
     b->closure_hi = closure_hi;
     b->mlp = closure_img;

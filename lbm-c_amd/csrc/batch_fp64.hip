@@ -9,7 +9,7 @@ hipError_t launch_batch_fp64(int lattice, const void *members, const int2 *chunk
                              int lds, hipStream_t s)
 {
     return launch_batch_st<StFP64>(lattice, members, chunks, nchunks, BatchShape{grid, sub, lds},
-                                   BatchEval{nullptr, 0, 0, nullptr, nullptr}, s);
+                                   BatchEval{nullptr, 0, 0, nullptr, nullptr, false}, s);
 }
 
 }  // namespace lbmc

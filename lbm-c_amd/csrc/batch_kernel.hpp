@@ -143,7 +143,7 @@ __global__ __launch_bounds__(kBlock) void batch_step_kernel(const BatchMember<ty
 // over whole waves, lanes past the member's last node evaluating a copy of
 // it and writing nothing.  The shared weights are staged into LDS once per
 // block.
-template <class L, class St>
+template <class L, class St, bool H>
 __global__ __launch_bounds__(kBlock, 4) void batch_eval_kernel(const BatchMember<float> *__restrict__ members,
                                                                const int2 *__restrict__ chunks, int nchunks,
                                                                const void *__restrict__ mlp, int sub,
@@ -151,14 +151,14 @@ __global__ __launch_bounds__(kBlock, 4) void batch_eval_kernel(const BatchMember
 {
     if (*any_eval == 0) return;
     __shared__ MlpEval w;
-    __shared__ mlp_h8 xb[kBlock];
+    __shared__ uint4 xb[kBlock];
     {
-        const uint4 *src = (const uint4 *)&((const MlpImage *)mlp)->ev;
+        const uint4 *src = (const uint4 *)eval_image<H>(mlp);
         uint4 *dst = (uint4 *)&w;
         for (int k = (int)threadIdx.x; k < (int)(sizeof(MlpEval) / 16); k += kBlock) dst[k] = src[k];
     }
     __syncthreads();
-    mlp_h8 *xw = xb + (threadIdx.x & ~63u);
+    uint4 *xw = xb + (threadIdx.x & ~63u);
     for (int c = (int)blockIdx.x; c < nchunks; c += (int)gridDim.x) {
         const int2 ch = chunks[c];
         if (ch.x < 0) continue;
@@ -177,11 +177,11 @@ __global__ __launch_bounds__(kBlock, 4) void batch_eval_kernel(const BatchMember
             float v[L::Q];
             gather_node<L, St, false>(a, n, dd, v);
             if (a.tmode == 1) {
-                if (variant == 9) closure_update<L, St, true, true>(a, n, live, v, w, xw);
-                else closure_update<L, St, false, true>(a, n, live, v, w, xw);
+                if (variant == 9) closure_update<L, St, true, true, H>(a, n, live, v, w, xw);
+                else closure_update<L, St, false, true, H>(a, n, live, v, w, xw);
             } else {
-                if (variant == 9) closure_update<L, St, true, false>(a, n, live, v, w, xw);
-                else closure_update<L, St, false, false>(a, n, live, v, w, xw);
+                if (variant == 9) closure_update<L, St, true, false, H>(a, n, live, v, w, xw);
+                else closure_update<L, St, false, false, H>(a, n, live, v, w, xw);
             }
             if (live) scatter_node<L, St, false>(a, n, dd, v);
         }
@@ -237,12 +237,14 @@ struct BatchShape {
 };
 
 // The closure members' evaluation: their chunk list (nchunks 0: no closure
-// members), its grid, the shared weights and the device flag
+// members), its grid, the shared weights, the device flag and the operand
+// precision (fp16: true, else bf16)
 struct BatchEval {
     const int2 *chunks;
     int nchunks, grid;
     const void *mlp;
     const int *any_eval;
+    bool fp16;
 };
 
 template <class L, class St>
@@ -253,10 +255,16 @@ hipError_t launch_batch_lattice(const void *members, const int2 *chunks, int nch
     hipLaunchKernelGGL((batch_step_kernel<L, St>), dim3(sh.grid), dim3(kBlock), sh.lds, s,
                        (const BatchMember<C> *)members, chunks, nchunks, sh.sub);
     if constexpr (std::is_same<C, float>::value) {
-        if (ev.nchunks > 0)
-            hipLaunchKernelGGL((batch_eval_kernel<L, St>), dim3(ev.grid), dim3(kBlock), 0, s,
-                               (const BatchMember<float> *)members, ev.chunks, ev.nchunks, ev.mlp, sh.sub,
-                               ev.any_eval);
+        if (ev.nchunks > 0) {
+            if (ev.fp16)
+                hipLaunchKernelGGL((batch_eval_kernel<L, St, true>), dim3(ev.grid), dim3(kBlock), 0, s,
+                                   (const BatchMember<float> *)members, ev.chunks, ev.nchunks, ev.mlp, sh.sub,
+                                   ev.any_eval);
+            else
+                hipLaunchKernelGGL((batch_eval_kernel<L, St, false>), dim3(ev.grid), dim3(kBlock), 0, s,
+                                   (const BatchMember<float> *)members, ev.chunks, ev.nchunks, ev.mlp, sh.sub,
+                                   ev.any_eval);
+        }
     }
     return hipGetLastError();
 }

@@ -21,9 +21,10 @@
 
 namespace lbmc {
 
-template <class L, class St, bool GUO, bool RECORD>
+// H: the network with fp16 operands (else bf16)
+template <class L, class St, bool GUO, bool RECORD, bool H>
 __device__ __forceinline__ void closure_update(const StepArgs<float> &a, int n, bool live, float (&v)[L::Q],
-                                               const MlpEval &w, mlp_h8 *xw)
+                                               const MlpEval &w, uint4 *xw)
 {
     constexpr int D = L::D;
     const unsigned m = a.all_fluid ? 0u : ldg(a.mask + n);
@@ -38,7 +39,7 @@ __device__ __forceinline__ void closure_update(const StepArgs<float> &a, int n, 
     fast_prepare<L, St::shifted, GUO>(v, F, st);
     float x[7];
     fast_closure_features<L, St::shifted>(st, x);
-    const float y = mlp_wave_eval(w, xw, mlp_input_h(w.sc, x));
+    const float y = mlp_wave_eval<H>(w, xw, mlp_input_eval<H>(w.sc, x));
     if (!live) return;
     if (m == 0) {
         float t = w.sc.out_a * y + w.sc.out_b + a.closure_shift;
@@ -90,18 +91,18 @@ __device__ __forceinline__ void closure_reuse_node(const StepArgs<float> &a, int
     }
 }
 
-template <class L, class St, bool GUO, bool RECORD, bool WIDE>
+template <class L, class St, bool GUO, bool RECORD, bool WIDE, bool H>
 __global__ __launch_bounds__(kBlock, 4) void closure_step_kernel(const StepArgs<float> a)
 {
     __shared__ MlpEval w;
-    __shared__ mlp_h8 xb[kBlock];
+    __shared__ uint4 xb[kBlock];
     {
-        const uint4 *src = (const uint4 *)&((const MlpImage *)a.mlp)->ev;
+        const uint4 *src = (const uint4 *)eval_image<H>(a.mlp);
         uint4 *dst = (uint4 *)&w;
         for (int k = (int)threadIdx.x; k < (int)(sizeof(MlpEval) / 16); k += kBlock) dst[k] = src[k];
     }
     __syncthreads();
-    mlp_h8 *xw = xb + (threadIdx.x & ~63u);
+    uint4 *xw = xb + (threadIdx.x & ~63u);
     // chunks in row_chunk order (the step kernels' XCD-aware order); the
     // grid is a multiple of 8 << rlog, so position p runs on XCD p mod 8
     const unsigned nchunks = tile_count(a.nnodes, kBlock);
@@ -116,7 +117,7 @@ __global__ __launch_bounds__(kBlock, 4) void closure_step_kernel(const StepArgs<
         slot_offsets<L>(a, n, dd);
         float v[L::Q];
         gather_node<L, St, WIDE>(a, n, dd, v);
-        closure_update<L, St, GUO, RECORD>(a, n, live, v, w, xw);
+        closure_update<L, St, GUO, RECORD, H>(a, n, live, v, w, xw);
         if (live) scatter_node<L, St, WIDE>(a, n, dd, v);
     }
 }

@@ -56,5 +56,7 @@ hipError_t launch_batch_fp64(int lattice, const void *members, const int2 *chunk
 // the fallback band, into device memory dst on stream s
 size_t mlp_image_bytes();
 lbmc_status mlp_build_image(const lbmc_mlp *m, void *dst, int device, double lo, double hi, hipStream_t s);
+// whether the model is evaluated with fp16 operands (lbmc_mlp_set_eval_fp16)
+bool mlp_eval_fp16(const lbmc_mlp *m);
 
 }  // namespace lbmc

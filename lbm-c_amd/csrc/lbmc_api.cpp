@@ -824,6 +824,7 @@ lbmc_status lbmc_set_closure(lbmc_ctx *c, const lbmc_mlp *model, double tau_lo, 
     if (!c) return fail(nullptr, LBMC_ERR_INVALID, "lbmc_set_closure: null context");
     if (!model) {
         c->key.closure = 0;
+        c->key.closure_fp16 = 0;
         c->closure_model = nullptr;
         drop_graph(c);
         return LBMC_OK;
@@ -841,6 +842,7 @@ lbmc_status lbmc_set_closure(lbmc_ctx *c, const lbmc_mlp *model, double tau_lo, 
     if (st != LBMC_OK) return fail(c, st, "lbmc_set_closure: model is on another device or the image build failed");
     HIP_TRY(c, hipStreamSynchronize(c->stream), "closure image build");
     c->key.closure = 1;
+    c->key.closure_fp16 = mlp_eval_fp16(model) ? 1 : 0;  // fixed until the model is set again
     c->closure_model = model;
     c->closure_lo = tau_lo;
     c->closure_hi = tau_hi;

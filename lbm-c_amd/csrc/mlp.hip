@@ -28,9 +28,44 @@ namespace lbmc {
 
 __device__ __forceinline__ __bf16 bf16_of(float x) { return __builtin_bit_cast(__bf16, bf16_bits(x)); }
 
-// Operand image from the flat FP32 parameters.  43 blocks of 64 lanes, one
-// block per operand array.
-constexpr int MLP_IMAGE_BLOCKS = 43;
+// 16-bit bits of a weight in the evaluation image: fp16 (H) or bf16
+template <bool H>
+__device__ __forceinline__ unsigned short eval_bits(float x)
+{
+    if constexpr (H) return __builtin_bit_cast(unsigned short, (_Float16)x);
+    else return bf16_bits(x);
+}
+
+// Block k < 12 of an evaluation image (32x32 products; fp16 operands when H)
+template <bool H>
+__device__ __forceinline__ void eval_image_block(int k, const float *p, MlpEval &ev)
+{
+    const int l = (int)threadIdx.x;
+    const float *W1 = p + MLP_W1, *W2 = p + MLP_W2, *b2 = p + MLP_B2, *w3 = p + MLP_W3;
+    if (k < 8) {
+        const int m = k >> 2, s = k & 3, h = l >> 5, c = l & 31;
+        unsigned short v[8];
+        for (int j = 0; j < 8; j++) v[j] = eval_bits<H>(W2[(32 * m + c) * 64 + mlp_hid32(s, 8 * h + j)]);
+        ev.w2[m][s][l] = __builtin_bit_cast(uint4, v);
+    } else if (k < 10) {
+        const int m = k - 8, h = l >> 5, c = l & 31;
+        unsigned short v[4];
+        for (int j = 0; j < 4; j++) v[j] = eval_bits<H>(W1[(32 * m + c) * 8 + 4 * h + j]);
+        ev.w1[m][l] = __builtin_bit_cast(uint2, v);
+    } else if (k == 10) {
+        const int m = l >> 5, h = (l >> 4) & 1, e = l & 15;
+        ev.b2[m][h][e] = b2[32 * m + mlp_row32(h, e)];
+    } else if (l < 32) {
+        const int h = l >> 4, m = (l >> 3) & 1, q = l & 7;
+        ev.w3[h][m][q] = (unsigned)eval_bits<H>(w3[32 * m + mlp_row32(h, 2 * q)]) |
+                         ((unsigned)eval_bits<H>(w3[32 * m + mlp_row32(h, 2 * q + 1)]) << 16);
+    }
+}
+
+// Operand image from the flat FP32 parameters.  55 blocks of 64 lanes: one
+// per operand array of training, 12 per evaluation image (bf16, fp16), and
+// the scalars.
+constexpr int MLP_IMAGE_BLOCKS = 55;
 
 __global__ __launch_bounds__(64) void mlp_image_kernel(const float *p, MlpScalars sc, MlpImage *img)
 {
@@ -68,42 +103,29 @@ __global__ __launch_bounds__(64) void mlp_image_kernel(const float *p, MlpScalar
         for (int j = 0; j < 4; j++)
             v[j] = __builtin_bit_cast(float, (unsigned)bf16_bits(w3[16 * h + 4 * g + j]) << 16);
         img->w3d[h][l] = v;
-    } else if (b < 38) {
-        // evaluation image (32x32 products, fp16 operands)
-        const int m = (b - 30) >> 2, s = (b - 30) & 3, h = l >> 5, c = l & 31;
-        mlp_h8 v;
-        for (int j = 0; j < 8; j++) v[j] = (_Float16)W2[(32 * m + c) * 64 + mlp_hid32(s, 8 * h + j)];
-        img->ev.w2[m][s][l] = v;
-    } else if (b < 40) {
-        const int m = b - 38, h = l >> 5, c = l & 31;
-        mlp_v4h v;
-        for (int j = 0; j < 4; j++) v[j] = (_Float16)W1[(32 * m + c) * 8 + 4 * h + j];
-        img->ev.w1[m][l] = v;
-    } else if (b == 40) {
-        const int m = l >> 5, h = (l >> 4) & 1, e = l & 15;
-        img->ev.b2[m][h][e] = b2[32 * m + mlp_row32(h, e)];
-    } else if (b == 41) {
-        if (l < 32) {
-            const int h = l >> 4, m = (l >> 3) & 1, q = l & 7;
-            const mlp_h2 v = {(_Float16)w3[32 * m + mlp_row32(h, 2 * q)], (_Float16)w3[32 * m + mlp_row32(h, 2 * q + 1)]};
-            img->ev.w3[h][m][q] = __builtin_bit_cast(unsigned, v);
-        }
+    } else if (b < 42) {
+        eval_image_block<false>(b - 30, p, img->ev);
+    } else if (b < 54) {
+        eval_image_block<true>(b - 42, p, img->evh);
     } else if (l == 0) {
         MlpScalars s = sc;
         s.b3 = p[MLP_B3];
         img->inf.sc = s;
         img->ev.sc = s;
+        img->evh.sc = s;
     }
 }
 
-// Network outputs of S samples (raw features, 7 per sample).
+// Network outputs of S samples (raw features, 7 per sample); fp16 operands
+// when H
+template <bool H>
 __global__ __launch_bounds__(256) void mlp_forward_kernel(const MlpImage *img, const float *feat, long long S,
                                                           float *y)
 {
     __shared__ MlpEval w;
-    __shared__ mlp_h8 xb[256];
+    __shared__ uint4 xb[256];
     {
-        const uint4 *src = (const uint4 *)&img->ev;
+        const uint4 *src = (const uint4 *)eval_image<H>(img);
         uint4 *dst = (uint4 *)&w;
         for (int k = (int)threadIdx.x; k < (int)(sizeof(MlpEval) / 16); k += 256) dst[k] = src[k];
     }
@@ -112,7 +134,7 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(const MlpImage *img, c
     const bool valid = i < S;
     float f[7];
     for (int k = 0; k < 7; k++) f[k] = valid ? feat[i * 7 + k] : 0.f;
-    const float out = mlp_wave_eval(w, xb + (threadIdx.x & ~63u), mlp_input_h(w.sc, f));
+    const float out = mlp_wave_eval<H>(w, xb + (threadIdx.x & ~63u), mlp_input_eval<H>(w.sc, f));
     if (valid) y[i] = out;
 }
 
@@ -296,6 +318,7 @@ struct lbmc_mlp {
     int pcap = 0;
     double *lossacc = nullptr;
     long long adam_t = 0;
+    bool eval_fp16 = false;  // evaluation with fp16 operands (default bf16)
     std::string err;
 };
 
@@ -413,6 +436,8 @@ lbmc_status mlp_build_image(const lbmc_mlp *m, void *dst, int device, double lo,
 
 size_t mlp_image_bytes() { return sizeof(MlpImage); }
 
+bool mlp_eval_fp16(const lbmc_mlp *m) { return m && m->eval_fp16; }
+
 }  // namespace lbmc
 
 extern "C" {
@@ -518,8 +543,12 @@ lbmc_status lbmc_mlp_forward(lbmc_mlp *m, const float *feat, long long S, float 
     hipError_t e = hipMalloc(&dy, (size_t)S * sizeof(float));
     if (e == hipSuccess) e = hipMemcpyAsync(dx, feat, (size_t)S * 7 * sizeof(float), hipMemcpyHostToDevice, m->stream);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(mlp_forward_kernel, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, m->stream, m->img,
-                           dx, S, dy);
+        if (m->eval_fp16)
+            hipLaunchKernelGGL(mlp_forward_kernel<true>, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, m->stream,
+                               m->img, dx, S, dy);
+        else
+            hipLaunchKernelGGL(mlp_forward_kernel<false>, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, m->stream,
+                               m->img, dx, S, dy);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(y, dy, (size_t)S * sizeof(float), hipMemcpyDeviceToHost, m->stream);
@@ -527,6 +556,13 @@ lbmc_status lbmc_mlp_forward(lbmc_mlp *m, const float *feat, long long S, float 
     (void)hipFree(dx);
     if (dy) (void)hipFree(dy);
     if (e != hipSuccess) return mfail(m, LBMC_ERR_DEVICE, "lbmc_mlp_forward: %s", hipGetErrorString(e));
+    return LBMC_OK;
+}
+
+lbmc_status lbmc_mlp_set_eval_fp16(lbmc_mlp *m, int on)
+{
+    if (!m) return mfail(m, LBMC_ERR_INVALID, "lbmc_mlp_set_eval_fp16: null model");
+    m->eval_fp16 = on != 0;
     return LBMC_OK;
 }
 

@@ -4,24 +4,27 @@
 // called from bgk_collision at :60-62): tau' = out_a * y + out_b, and the
 // Smagorinsky tau is used wherever tau' leaves the band [lo, hi].
 //
-// Arithmetic: training (mlp_tile_forward and the backward pass) takes bf16
-// operands on 16x16 tiles (MFMA 16x16x16 bf16 for the first layer, 16x16x32
+// Arithmetic: bf16 operands on the matrix cores, FP32 accumulation, biases
+// and derivatives in FP32.  Training (mlp_tile_forward and the backward
+// pass) uses 16x16 tiles (MFMA 16x16x16 bf16 for the first layer, 16x16x32
 // bf16 for the others); evaluation in the closure steps (mlp_wave_eval)
-// takes fp16 operands on 32x32 tiles (32x32x8 and 32x32x16 f16, the output
-// layer as fp16 dot products).  Accumulation, biases and derivatives are
-// FP32.  GELU(z) = z Phi(z), Phi from an odd degree-9 polynomial
-// (least-squares fit of the normal CDF on [-3.5, 3.5], |error| <= 8.6e-4 in
-// exact arithmetic), clamped to [0, 1].  The polynomial runs in packed FP16
-// (gelu_pair, gelu_oct): x = fp16(z), t = x*x, four fused multiply-adds, the
-// [0, 1] clamp folded into the last one.  Training then forms z * Phi in
-// FP32 and rounds it to bf16 (10 vector instructions per pair of
-// activations against 17 for the same polynomial in FP32); evaluation forms
-// x * Phi in packed fp16, the next product's operand as it stands (8 per
-// pair).  Over every finite fp16 x this Phi is within 1.8e-3 of the normal
-// CDF and exactly 0 / 1 beyond -3.5 / 3.5, so the input needs no clamp;
-// |GELU error| <= 5.2e-3 (bf16 result) and the half-ulp of the fp16 result
-// is 2^-11 relative.  An fp16 pre-activation beyond 65504 makes x infinite:
-// the evaluation's GELU is then inf or NaN and tau' leaves its band (the
+// uses 32x32 tiles (32x32x8 and 32x32x16 bf16, the output layer as bf16 dot
+// products).  A model may instead be evaluated with fp16 operands (the same
+// tiles with f16 products: mlp_wave_eval<true>), which saves two vector
+// instructions per pair of activations.  GELU(z) = z Phi(z), Phi from an
+// odd degree-9 polynomial (least-squares fit of the normal CDF on
+// [-3.5, 3.5], |error| <= 8.6e-4 in exact arithmetic), clamped to [0, 1].
+// The polynomial runs in packed FP16 (gelu_pair, gelu_oct): x = fp16(z),
+// t = x*x, four fused multiply-adds, the [0, 1] clamp folded into the last
+// one; then z * Phi in FP32 and the bf16 rounding the next product needs
+// (10 vector instructions per pair against 17 for the same polynomial in
+// FP32), or with fp16 operands x * Phi in packed fp16, the next product's
+// operand as it stands (8 per pair).  Over every finite fp16 x this Phi is
+// within 1.8e-3 of the normal CDF and exactly 0 / 1 beyond -3.5 / 3.5, so
+// the input needs no clamp; |GELU error| <= 5.2e-3, below the bf16 half-ulp
+// of the result (2^-9 relative); the fp16 result adds half an fp16 ulp
+// (2^-11 relative).  With fp16 operands a pre-activation beyond 65504 makes
+// x infinite: the GELU is then inf or NaN and tau' leaves its band (the
 // Smagorinsky tau is used).
 //
 // Training's wave mapping (64 lanes, lane l = 16 g + r): a wave takes 64
@@ -90,13 +93,14 @@ __host__ __device__ constexpr int mlp_hid32(int s, int k)
     return 16 * s + 8 * ((k & 7) >> 2) + 4 * (k >> 3) + (k & 3);
 }
 
-// Operand-ordered fp16 weights of the closure's evaluation (mlp_wave_eval):
-// 32x32 products, lane l = 32h + c.
+// Operand-ordered weights of the closure's evaluation (mlp_wave_eval):
+// 32x32 products, lane l = 32h + c; 16-bit values, bf16 or fp16 by image
+// (MlpImage::ev, evh), held as raw bits.
 struct MlpEval {
-    mlp_h8 w2[2][4][64];   // A of layer 2, row tile m, K-step s: W2[32m + c][hid32(s, 8h + j)]
-    mlp_v4h w1[2][64];     // A of layer 1, row tile m: W1[32m + c][4h + j]
+    uint4 w2[2][4][64];    // A of layer 2, row tile m, K-step s: W2[32m + c][hid32(s, 8h + j)]
+    uint2 w1[2][64];       // A of layer 1, row tile m: W1[32m + c][4h + j]
     mlp_v16f b2[2][2];     // layer-2 accumulator start, row tile m, lane half h: b2[32m + row32(h, e)]
-    unsigned w3[2][2][8];  // fp16 pairs (low, high) of w3[32m + row32(h, 2q)], w3[32m + row32(h, 2q + 1)], [h][m][q]
+    unsigned w3[2][2][8];  // pairs (low, high) of w3[32m + row32(h, 2q)], w3[32m + row32(h, 2q + 1)], [h][m][q]
     MlpScalars sc;
 };
 
@@ -105,14 +109,21 @@ struct MlpImage {
     // backward only
     mlp_v8b w2t[4][2][64];  // A of W2^T, row tile h, chunk c: W2[hid(c, g, j)][16h + r]
     mlp_v4f w3d[4][64];     // w3[16h + 4g + j] as rounded to bf16, in FP32
-    // evaluation (closure steps and lbmc_mlp_forward)
-    MlpEval ev;
+    // evaluation (closure steps and lbmc_mlp_forward): bf16 and fp16 operands
+    MlpEval ev, evh;
 };
 
 static_assert(sizeof(MlpInfer) % 16 == 0, "MlpInfer is copied in 16-byte units");
 static_assert(sizeof(MlpEval) % 16 == 0, "MlpEval is copied in 16-byte units");
 static_assert(sizeof(MlpImage) % 16 == 0, "MlpImage is copied in 16-byte units");
-static_assert(offsetof(MlpImage, ev) % 64 == 0, "MlpEval holds 64-byte vectors");
+static_assert(offsetof(MlpImage, ev) % 64 == 0 && offsetof(MlpImage, evh) % 64 == 0, "MlpEval holds 64-byte vectors");
+
+// the evaluation image of a model: fp16 operands (H) or bf16
+template <bool H>
+__host__ __device__ inline const MlpEval *eval_image(const void *img)
+{
+    return H ? &((const MlpImage *)img)->evh : &((const MlpImage *)img)->ev;
+}
 
 // GELU coefficients: Phi(x) - 1/2 = x (c1 + t (c3 + t (c5 + t (c7 + t c9)))), t = x^2
 constexpr float GELU_C1 = 0.3964735269546509f;
@@ -258,14 +269,27 @@ __device__ __forceinline__ void mlp_tile_forward(const MlpInfer &w, const mlp_v8
     o.y = yv[0];
 }
 
-__device__ __forceinline__ mlp_v16f mfma_32k8(mlp_v4h a, mlp_v4h b)
+// 32x32 products of 16-bit operands held as raw bits: fp16 (H) or bf16
+template <bool H>
+__device__ __forceinline__ mlp_v16f mfma_32k8(uint2 a, uint2 b)
 {
-    return __builtin_amdgcn_mfma_f32_32x32x8f16(a, b, mlp_v16f{}, 0, 0, 0);
+    if constexpr (H)
+        return __builtin_amdgcn_mfma_f32_32x32x8f16(__builtin_bit_cast(mlp_v4h, a), __builtin_bit_cast(mlp_v4h, b),
+                                                    mlp_v16f{}, 0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(__builtin_bit_cast(mlp_v4s, a),
+                                                        __builtin_bit_cast(mlp_v4s, b), mlp_v16f{}, 0, 0, 0);
 }
 
-__device__ __forceinline__ mlp_v16f mfma_32k16(mlp_h8 a, mlp_h8 b, mlp_v16f c)
+template <bool H>
+__device__ __forceinline__ mlp_v16f mfma_32k16(uint4 a, uint4 b, mlp_v16f c)
 {
-    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    if constexpr (H)
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(mlp_h8, a), __builtin_bit_cast(mlp_h8, b), c,
+                                                      0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(mlp_v8b, a), __builtin_bit_cast(mlp_v8b, b),
+                                                       c, 0, 0, 0);
 }
 
 __device__ __forceinline__ mlp_h8 mlp_h8_splat(float c)
@@ -274,10 +298,12 @@ __device__ __forceinline__ mlp_h8 mlp_h8_splat(float c)
     return mlp_h8{h, h, h, h, h, h, h, h};
 }
 
-// GELU of z[e0 .. e0 + 7] as the fp16 operand of the next product: the Phi
-// of gelu_pair (four pairs whose packed operations issue side by side), then
-// x * Phi rounded to fp16 once, x = fp16(z); element j = z[e0 + j]
-__device__ __forceinline__ mlp_h8 gelu_oct(const mlp_v16f &z, int e0)
+// GELU of z[e0 .. e0 + 7] as the next product's operand, element j =
+// z[e0 + j]: the Phi of gelu_pair (four pairs whose packed operations issue
+// side by side), then z * Phi rounded to bf16 as in gelu_pair, or with fp16
+// operands (H) x * Phi rounded to fp16 once, x = fp16(z)
+template <bool H>
+__device__ __forceinline__ uint4 gelu_oct(const mlp_v16f &z, int e0)
 {
     mlp_h8 x;
 #pragma unroll
@@ -289,45 +315,65 @@ __device__ __forceinline__ mlp_h8 gelu_oct(const mlp_v16f &z, int e0)
     p = __builtin_elementwise_fma(t, p, mlp_h8_splat(GELU_C1));
     mlp_h8 s = __builtin_elementwise_fma(x, p, mlp_h8_splat(0.5f));
     s = __builtin_elementwise_min(__builtin_elementwise_max(s, mlp_h8_splat(0.f)), mlp_h8_splat(1.f));
-    return x * s;
-}
-
-// sum over the four fp16 pairs of a and b, added to c in FP32
-__device__ __forceinline__ float dot8(mlp_h8 a, uint4 b, float c)
-{
-    const uint4 ua = __builtin_bit_cast(uint4, a);
-    c = __builtin_amdgcn_fdot2(__builtin_bit_cast(mlp_h2, ua.x), __builtin_bit_cast(mlp_h2, b.x), c, false);
-    c = __builtin_amdgcn_fdot2(__builtin_bit_cast(mlp_h2, ua.y), __builtin_bit_cast(mlp_h2, b.y), c, false);
-    c = __builtin_amdgcn_fdot2(__builtin_bit_cast(mlp_h2, ua.z), __builtin_bit_cast(mlp_h2, b.z), c, false);
-    return __builtin_amdgcn_fdot2(__builtin_bit_cast(mlp_h2, ua.w), __builtin_bit_cast(mlp_h2, b.w), c, false);
-}
-
-// a sample's raw features -> normalised fp16 input row of the evaluation
-// (constant-1 bias input last)
-__device__ __forceinline__ mlp_h8 mlp_input_h(const MlpScalars &s, const float (&f)[7])
-{
-    mlp_h8 r;
+    if constexpr (H) {
+        return __builtin_bit_cast(uint4, x * s);
+    } else {
+        mlp_v8b r;
 #pragma unroll
-    for (int k = 0; k < 7; k++) r[k] = (_Float16)((f[k] - s.mu[k]) * s.isd[k]);
-    r[7] = (_Float16)1.f;
-    return r;
+        for (int j = 0; j < 8; j++) r[j] = (__bf16)__builtin_fmaf(z[e0 + j], (float)s[j], 0.f);
+        return __builtin_bit_cast(uint4, r);
+    }
+}
+
+// sum over the four 16-bit pairs of a and b (fp16 or bf16), added to c in FP32
+template <bool H>
+__device__ __forceinline__ float dot8(uint4 a, uint4 b, float c)
+{
+    const unsigned av[4] = {a.x, a.y, a.z, a.w}, bv[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        if constexpr (H)
+            c = __builtin_amdgcn_fdot2(__builtin_bit_cast(mlp_h2, av[q]), __builtin_bit_cast(mlp_h2, bv[q]), c, false);
+        else
+            c = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(mlp_b2, av[q]), __builtin_bit_cast(mlp_b2, bv[q]),
+                                                c, false);
+    }
+    return c;
+}
+
+// a sample's raw features -> normalised input row of the evaluation (bf16,
+// or fp16 when H; the constant-1 bias input last)
+template <bool H>
+__device__ __forceinline__ uint4 mlp_input_eval(const MlpScalars &s, const float (&f)[7])
+{
+    if constexpr (H) {
+        mlp_h8 r;
+#pragma unroll
+        for (int k = 0; k < 7; k++) r[k] = (_Float16)((f[k] - s.mu[k]) * s.isd[k]);
+        r[7] = (_Float16)1.f;
+        return __builtin_bit_cast(uint4, r);
+    } else {
+        return __builtin_bit_cast(uint4, mlp_input(s, f));
+    }
 }
 
 // Network output for the lane's own sample (input row xin), from the
-// evaluation image.  xb: the wave's 64-entry LDS staging area.  Called by all
-// 64 lanes together.
+// evaluation image of the operand type (bf16, or fp16 when H).  xb: the
+// wave's 64-entry LDS staging area.  Called by all 64 lanes together.
 //
 // The wave's 64 samples are two tiles of 32 (lane l = 32h + c holds column c
 // of a tile), each layer again computed transposed with the weights as A:
 // layer 1 is two 32x32x8 products (K = the 7 inputs and the bias input),
 // layer 2 two row tiles of four 32x32x16 products whose K order mlp_hid32 is
 // the order in which the lanes hold layer 1's GELUs, and the output layer a
-// dot product of each lane's 32 layer-2 GELUs with w3 (fp16 pairs, FP32 sum)
-// plus the other lane half's.  Against the 16x16 tiles of mlp_tile_forward
-// (training) this skips the zero K rows of layer 1 and the 15 unused rows of
-// the output layer's product, a quarter of the matrix-core work, and reads
-// half the weight operands per sample.
-__device__ __forceinline__ float mlp_wave_eval(const MlpEval &w, mlp_h8 *xb, mlp_h8 xin)
+// dot product of each lane's 32 layer-2 GELUs with w3 (16-bit pairs, FP32
+// sum) plus the other lane half's.  Against the 16x16 tiles of
+// mlp_tile_forward (training) this skips the zero K rows of layer 1 and the
+// 15 unused rows of the output layer's product, a quarter of the
+// matrix-core work, and reads half the weight operands per sample; with
+// bf16 operands only the FP32 summation order differs from training.
+template <bool H>
+__device__ __forceinline__ float mlp_wave_eval(const MlpEval &w, uint4 *xb, uint4 xin)
 {
     const int l = (int)(threadIdx.x & 63), h = l >> 5, c = l & 31;
     xb[l] = xin;
@@ -340,15 +386,15 @@ __device__ __forceinline__ float mlp_wave_eval(const MlpEval &w, mlp_h8 *xb, mlp
 #pragma unroll 1
     for (int t = 0; t < 2; t++) {
         // layer-1 B operand: inputs 4h..4h+3 of sample 32t + c
-        const mlp_v4h xop = ((const mlp_v4h *)&xb[32 * t + c])[h];
+        const uint2 xop = ((const uint2 *)&xb[32 * t + c])[h];
         // one 32-row tile at a time, phases fenced from the scheduler: the
         // node's own state stays in registers beside the network's
-        mlp_h8 a[4];
+        uint4 a[4];
 #pragma unroll
         for (int m = 0; m < 2; m++) {
-            const mlp_v16f z = mfma_32k8(w.w1[m][lw], xop);
-            a[2 * m] = gelu_oct(z, 0);
-            a[2 * m + 1] = gelu_oct(z, 8);
+            const mlp_v16f z = mfma_32k8<H>(w.w1[m][lw], xop);
+            a[2 * m] = gelu_oct<H>(z, 0);
+            a[2 * m + 1] = gelu_oct<H>(z, 8);
             __builtin_amdgcn_sched_barrier(0);
         }
         float part = 0.f;
@@ -356,11 +402,11 @@ __device__ __forceinline__ float mlp_wave_eval(const MlpEval &w, mlp_h8 *xb, mlp
         for (int m = 0; m < 2; m++) {
             mlp_v16f acc = w.b2[m][hw];
 #pragma unroll
-            for (int s = 0; s < 4; s++) acc = mfma_32k16(w.w2[m][s][lw], a[s], acc);
+            for (int s = 0; s < 4; s++) acc = mfma_32k16<H>(w.w2[m][s][lw], a[s], acc);
             __builtin_amdgcn_sched_barrier(0);
             const uint4 *w3 = (const uint4 *)w.w3[hw][m];
-            part = dot8(gelu_oct(acc, 0), w3[0], part);
-            part = dot8(gelu_oct(acc, 8), w3[1], part);
+            part = dot8<H>(gelu_oct<H>(acc, 0), w3[0], part);
+            part = dot8<H>(gelu_oct<H>(acc, 8), w3[1], part);
             __builtin_amdgcn_sched_barrier(0);
         }
         // both lane halves hold partial sums of sample 32t + c

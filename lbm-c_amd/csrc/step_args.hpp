@@ -31,6 +31,7 @@ struct KernelKey {
     int mfix;     // corrected D3Q15 M^-1 (MRT family only)
     int wide;     // a direction array spans 4 GiB or more: 64-bit offsets
     int closure;  // learned relaxation-time closure (BGK family, FP32 compute)
+    int closure_fp16;  // its network with fp16 operands (else bf16)
     int lds;      // dynamic LDS bytes per block of the step kernels (which use
                   //   none): caps the blocks resident per CU, 0 = no cap
     int pairs;    // two nodes per lane: on every call in the half-interleaved 16-bit layout

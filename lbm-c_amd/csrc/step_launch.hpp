@@ -43,23 +43,27 @@ static hipError_t launch_one(const StepArgs<typename St::C> &a, int grid, hipStr
 // record (grid: the persistent closure grid for 0/1, one block per 256 nodes
 // for 2; lds: the step kernels' residency cap, applied to the reuse kernel,
 // which has their memory pattern)
-template <class L, class St, bool GUO, bool W>
+template <class L, class St, bool GUO, bool W, bool H>
 static hipError_t launch_closure_w(const StepArgs<float> &a, int grid, hipStream_t s, int lds)
 {
     if (a.tmode == 2)
         hipLaunchKernelGGL((closure_reuse_kernel<L, St, GUO, W>), dim3(grid), dim3(kBlock), lds, s, a);
     else if (a.tmode == 1)
-        hipLaunchKernelGGL((closure_step_kernel<L, St, GUO, true, W>), dim3(grid), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL((closure_step_kernel<L, St, GUO, true, W, H>), dim3(grid), dim3(kBlock), 0, s, a);
     else
-        hipLaunchKernelGGL((closure_step_kernel<L, St, GUO, false, W>), dim3(grid), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL((closure_step_kernel<L, St, GUO, false, W, H>), dim3(grid), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }
 
+// h: the network with fp16 operands (else bf16)
 template <class L, class St, bool GUO>
-static hipError_t launch_closure(const StepArgs<float> &a, int grid, hipStream_t s, bool wide, int lds)
+static hipError_t launch_closure(const StepArgs<float> &a, int grid, hipStream_t s, bool wide, bool h, int lds)
 {
-    return wide ? launch_closure_w<L, St, GUO, true>(a, grid, s, lds)
-                : launch_closure_w<L, St, GUO, false>(a, grid, s, lds);
+    if (h)
+        return wide ? launch_closure_w<L, St, GUO, true, true>(a, grid, s, lds)
+                    : launch_closure_w<L, St, GUO, false, true>(a, grid, s, lds);
+    return wide ? launch_closure_w<L, St, GUO, true, false>(a, grid, s, lds)
+                : launch_closure_w<L, St, GUO, false, false>(a, grid, s, lds);
 }
 
 template <class L, class St>
@@ -68,8 +72,8 @@ static hipError_t launch_lattice(const KernelKey &k, const StepArgs<typename St:
 {
     if constexpr (std::is_same<typename St::C, float>::value) {
         if (k.closure && k.family == 0) {
-            return k.guo ? launch_closure<L, St, true>(a, grid, s, k.wide != 0, k.lds)
-                         : launch_closure<L, St, false>(a, grid, s, k.wide != 0, k.lds);
+            return k.guo ? launch_closure<L, St, true>(a, grid, s, k.wide != 0, k.closure_fp16 != 0, k.lds)
+                         : launch_closure<L, St, false>(a, grid, s, k.wide != 0, k.closure_fp16 != 0, k.lds);
         }
     }
     if (k.family == 0) {

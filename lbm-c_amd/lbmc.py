@@ -38,7 +38,8 @@ EXPORTS = [
     "lbmc_get_f_noalloc",
     "lbmc_kinetic_energy", "lbmc_sync", "lbmc_info", "lbmc_copy_bandwidth",
     "lbmc_mlp_create", "lbmc_mlp_destroy", "lbmc_mlp_last_error", "lbmc_mlp_set_params",
-    "lbmc_mlp_get_params", "lbmc_mlp_set_scaling", "lbmc_mlp_forward", "lbmc_mlp_loss_grad",
+    "lbmc_mlp_get_params", "lbmc_mlp_set_scaling", "lbmc_mlp_forward", "lbmc_mlp_set_eval_fp16",
+    "lbmc_mlp_loss_grad",
     "lbmc_mlp_set_data", "lbmc_mlp_train", "lbmc_set_closure", "lbmc_set_closure_interval",
     "lbmc_set_closure_shift",
     "lbmc_batch_create", "lbmc_batch_destroy", "lbmc_batch_last_error", "lbmc_batch_run",
@@ -118,6 +119,7 @@ def _load():
         "lbmc_mlp_get_params": (ct.c_int, [P, fp]),
         "lbmc_mlp_set_scaling": (ct.c_int, [P, ct.POINTER(MlpScaling)]),
         "lbmc_mlp_forward": (ct.c_int, [P, fp, ct.c_longlong, fp]),
+        "lbmc_mlp_set_eval_fp16": (ct.c_int, [P, ct.c_int]),
         "lbmc_mlp_loss_grad": (ct.c_int, [P, fp, fp, ct.c_longlong, fp, ct.POINTER(ct.c_double)]),
         "lbmc_mlp_set_data": (ct.c_int, [P, fp, fp, ct.c_longlong]),
         "lbmc_mlp_train": (ct.c_int, [P, ct.c_int, ct.c_int, ct.c_float, ct.c_ulonglong,
@@ -425,6 +427,11 @@ class Mlp:
         p = np.empty(MLP_NPARAM, dtype=np.float32)
         self._check(_lib.lbmc_mlp_get_params(self._h, _fptr(p)))
         return p
+
+    def set_eval_fp16(self, on=True):
+        """Evaluate with fp16 operands instead of bf16 (forward, and the
+        closure steps of lattices that attach the model afterwards)."""
+        self._check(_lib.lbmc_mlp_set_eval_fp16(self._h, 1 if on else 0))
 
     def forward(self, feat):
         x = _f32(feat).reshape(-1, 7)

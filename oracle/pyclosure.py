@@ -8,7 +8,8 @@ optional uniform Guo force, the reference's call semantics
 (LBM-C/solver.cu:10-136): call 0 collides in place, every later call first
 pulls f_i(x) <- f_i(x - e_i) and then collides; solids bounce back
 (LBM-C/collision.cu:423-443).  At fluid nodes tau comes from the network
-(oracle/pymlp.py, the evaluation's fp16 quantisation points emulated) as out_a * y + out_b
+(oracle/pymlp.py, the evaluation's bf16 -- or with fp16=True fp16 --
+quantisation points emulated) as out_a * y + out_b
 + shift (shift: the lattice's offset, 0 by default; evaluated on calls t % every == 0 and the first call, the recorded tau
 reused in between), or from the Smagorinsky formula (LBM-C/collision.cu:445-463, restated in
 closed form: tau' = (tau + sqrt(tau^2 + 36 C^2 (sum|Pi_aa| + 2 sum_{a<b}
@@ -27,7 +28,7 @@ import pyoracle
 
 class ClosureOracle:
     def __init__(self, lattice, length, tau, c_smag, params, mu, isd, out_a, out_b, lo, hi,
-                 force=None, solid=None, quant=True, every=1, shift=0.0):
+                 force=None, solid=None, quant=True, every=1, shift=0.0, fp16=False):
         self.lattice = lattice
         self.dim = 2 if lattice == pyoracle.D2Q9 else 3
         e, w, opp, _, _ = pyoracle.tables(lattice)
@@ -47,7 +48,7 @@ class ClosureOracle:
         self.guo = force is not None
         n = int(np.prod(self.shape))
         self.solid = np.zeros(n, dtype=bool) if solid is None else np.asarray(solid, dtype=bool).reshape(-1)
-        self.quant = quant
+        self.quant = ("fp16" if fp16 else True) if quant else False
         self.every = int(every)
         self.record = None   # tau of the last evaluation per node, NaN where it fell back
         self.f = None
@@ -103,7 +104,7 @@ class ClosureOracle:
         rho, u, feq, pi, x = self.features(f)
         if self.every <= 1 or self.record is None or self.calls % self.every == 0:
             xn = ((x.astype(np.float32) - self.mu) * self.isd).astype(np.float64)
-            y, _ = pymlp.forward(self.p, xn, quant="eval" if self.quant else False)
+            y, _ = pymlp.forward(self.p, xn, quant=self.quant)
             t = self.out_a * y + self.out_b + self.shift
             self.tau_net = t
             bad = ~((t >= self.lo) & (t <= self.hi))

@@ -14,23 +14,24 @@ normal CDF on [-3.5, 3.5]) added to 1/2 and clamped to [0, 1].
     multiply-add clamped to [0, 1], no input clamp; GELU = fp32(z * Phi).
     |error| <= 1.8e-3 in Phi over every finite fp16 input, exactly 0 / 1
     beyond -3.5 / 3.5.
-  * gelu16h: the evaluation's GELU (mlp.hpp gelu_oct): the same Phi, then
-    fp16(x * Phi) with x = fp16(z).
+  * gelu16h: the GELU of the evaluation with fp16 operands (mlp.hpp
+    gelu_oct<true>): the same Phi, then fp16(x * Phi) with x = fp16(z).
 tests/test_closure_math.py checks all of these against erf.
 
 Parameters (one dict of float64 arrays):
     w1 (64, 8)  hidden x (7 normalised inputs + a constant-1 bias column)
     w2 (64, 64), b2 (64,), w3 (64,), b3 ()
 
-`quant=True` is the training kernels' arithmetic (lbm-c_amd/csrc/mlp.hip
-mlp_train_kernel): activations and their derivatives as the device
-evaluates them (gelu16, dgelu16), and the inputs, the weights, both hidden
-activations and both hidden-layer error signals rounded to bfloat16 (round
-to nearest even, through float32) before they enter a product.
-`quant="eval"` is the evaluation's (mlp.hpp mlp_wave_eval: the closure
-steps and lbmc_mlp_forward): inputs and weights rounded to fp16 through
-float32, activations gelu16h (already fp16), layer-2 bias in float32.  Sums
-stay in float64.  The device accumulates in float32, so quantised results
+`quant=True` is the device's bf16 arithmetic, the training kernels'
+(lbm-c_amd/csrc/mlp.hip mlp_train_kernel) and the default evaluation's
+(mlp.hpp mlp_wave_eval: the closure steps and lbmc_mlp_forward):
+activations and their derivatives as the device evaluates them (gelu16,
+dgelu16), and the inputs, the weights, both hidden activations and both
+hidden-layer error signals rounded to bfloat16 (round to nearest even,
+through float32) before they enter a product.  `quant="fp16"` is the
+evaluation with fp16 operands (lbmc_mlp_set_eval_fp16): inputs and weights
+rounded to fp16 through float32, activations gelu16h (already fp16),
+layer-2 bias in float32.  Sums stay in float64.  The device accumulates in float32, so quantised results
 agree with the device to float32 summation error (and the rare fp16 / bf16
 rounding that lands on the other side of a tie because of it).
 """
@@ -163,8 +164,8 @@ def unflatten(v):
 
 def forward(p, x, quant=False):
     """x: (S, 7) normalised features -> y (S,), cache.  quant: False (exact),
-    True (training arithmetic) or "eval" (evaluation arithmetic)."""
-    if quant == "eval":
+    True (bf16 arithmetic) or "fp16" (the evaluation with fp16 operands)."""
+    if quant == "fp16":
         X = np.concatenate([x, np.ones((x.shape[0], 1))], axis=1)
         Xq, W1, W2, w3 = fp16(X), fp16(p["w1"]), fp16(p["w2"]), fp16(p["w3"])
         z1 = Xq @ W1.T

@@ -134,7 +134,7 @@ def test_quantised_forward_close_to_exact():
     assert np.max(np.abs(y1 - y0)) < 0.05 * np.max(np.abs(y0))
     # the evaluation's fp16 operands carry 3 more mantissa bits than bf16:
     # its rounding error is the smaller, the GELU approximation common to both
-    y2, _ = pymlp.forward(p, x, quant="eval")
+    y2, _ = pymlp.forward(p, x, quant="fp16")
     assert np.max(np.abs(y2 - y0)) < 0.05 * np.max(np.abs(y0))
     # the same network with the device's GELU and no operand rounding
     X = np.concatenate([x, np.ones((x.shape[0], 1))], axis=1)

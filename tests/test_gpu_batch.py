@@ -258,13 +258,14 @@ def test_batch_rejects_mixed_members():
         x.close()
 
 
-@pytest.mark.parametrize("every", [1, 3])
-def test_batch_with_closure_members_matches_single_lattices(every):
-    """D2Q9 members with the learned closure (one shared network; with and
-    without Guo forcing; the network evaluated every call or every 3rd, with
-    the recorded tau in between) beside an MRT member: each ends where it
-    ends when stepped on its own, over two runs (the second starting from a
-    valid record at a call that is not a multiple of 3)."""
+@pytest.mark.parametrize("every,fp16", [(1, False), (3, False), (1, True)], ids=["every1", "every3", "every1-fp16"])
+def test_batch_with_closure_members_matches_single_lattices(every, fp16):
+    """D2Q9 members with the learned closure (one shared network, bf16 or
+    fp16 operands; with and without Guo forcing; the network evaluated every
+    call or every 3rd, with the recorded tau in between) beside an MRT
+    member: each ends where it ends when stepped on its own, over two runs
+    (the second starting from a valid record at a call that is not a
+    multiple of 3)."""
     import pymlp
     import workloads
 
@@ -277,6 +278,8 @@ def test_batch_with_closure_members_matches_single_lattices(every):
     isd = np.full(7, 300.0, dtype=np.float32)
     isd[6] = 100.0
     model = lbmc.Mlp(pymlp.flatten(p), mu=mu, isd=isd, out_a=0.02, out_b=0.62)
+    if fp16:
+        model.set_eval_fp16(True)
     specs = []
     for k, (Lx, Ly) in enumerate([(40, 24), (72, 33), (17, 50)]):
         geo = workloads.channel(Lx, Ly)

@@ -1,9 +1,10 @@
 """Learned relaxation-time closure on the GPU against the FP64 oracles
 (oracle/pymlp.py for the network, oracle/pyclosure.py for the step):
-forward values of the evaluation (fp16 rounding points emulated in the
-oracle), loss and gradient of the training kernels (bf16 rounding points
-emulated), FP32 accumulation on the device; the Adam step, training
-progress and determinism, and the fused closure step."""
+forward values of the evaluation (bf16 rounding points emulated in the
+oracle, and fp16 ones for lbmc_mlp_set_eval_fp16), loss and gradient of the
+training kernels (bf16 rounding points emulated), FP32 accumulation on the
+device; the Adam step, training progress and determinism, and the fused
+closure step with either evaluation."""
 import numpy as np
 import pytest
 
@@ -25,8 +26,9 @@ def random_params(seed, bias=0.1):
     return p
 
 
-def test_forward_matches_oracle():
-    print("[closure] forward S=1000", flush=True)
+@pytest.mark.parametrize("fp16", [False, True], ids=["bf16", "fp16"])
+def test_forward_matches_oracle(fp16):
+    print(f"[closure] forward S=1000 fp16={fp16}", flush=True)
     rng = np.random.default_rng(10)
     p = random_params(10)
     S = 1000  # not a multiple of 64: the last wave is ragged
@@ -34,18 +36,20 @@ def test_forward_matches_oracle():
     isd = (0.5 + rng.random(7)).astype(np.float32)
     feat = (mu + rng.standard_normal((S, 7)) / isd).astype(np.float32)
     m = lbmc.Mlp(pymlp.flatten(p), mu=mu, isd=isd)
+    if fp16:
+        m.set_eval_fp16(True)
     y = m.forward(feat).astype(np.float64)
     xn = ((feat - mu) * isd).astype(np.float64)
-    yq, _ = pymlp.forward(p, xn, quant="eval")
+    yq, _ = pymlp.forward(p, xn, quant="fp16" if fp16 else True)
     y0, _ = pymlp.forward(p, xn, quant=False)
     scale = np.max(np.abs(yq))
     err = np.abs(y - yq)
     print(f"[closure] forward: median {np.median(err) / scale:.2e}, max {np.max(err) / scale:.2e} of max|y|",
           flush=True)
-    # FP32 accumulation vs FP64 with the same fp16 operands and activation
+    # FP32 accumulation vs FP64 with the same operands and activation
     # arithmetic: typical differences are FP32 summation error (~1e-8 of
-    # max|y|); an fp16 rounding that lands on the other side of a tie moves
-    # one activation by one ulp, which bounds the worst sample
+    # max|y|); an fp16 or bf16 rounding that lands on the other side of a
+    # tie moves one activation by one ulp, which bounds the worst sample
     assert np.median(err) < 1e-6 * scale
     assert np.max(err) < 1e-3 * scale
     # and the quantised network stays close to the exact one
@@ -141,8 +145,10 @@ MU = np.array([0, 0, 0, 0, 0, 0, 1.0], dtype=np.float32)
 ISD = np.array([300.0] * 6 + [100.0], dtype=np.float32)
 
 
-def _pair(lattice, L, tau, c_smag, storage, band, out, force=None, solid=None, seed=20, every=1, shift=0.0):
-    """The GPU lattice and the oracle, same initial state and network."""
+def _pair(lattice, L, tau, c_smag, storage, band, out, force=None, solid=None, seed=20, every=1, shift=0.0,
+          fp16=False):
+    """The GPU lattice and the oracle, same initial state and network (fp16:
+    evaluated with fp16 operands)."""
     dim = 2 if lattice == lbmc.D2Q9 else 3
     n = int(np.prod(L))
     rho, u = _initial_state(lattice, L, seed)
@@ -157,25 +163,28 @@ def _pair(lattice, L, tau, c_smag, storage, band, out, force=None, solid=None, s
     lat.set_fields(**fields)
     lat.init_equilibrium(rho, u)
     m = lbmc.Mlp(pymlp.flatten(p), mu=MU, isd=ISD, out_a=out[0], out_b=out[1])
+    if fp16:
+        m.set_eval_fp16(True)
     lat.set_closure(m, band[0], band[1])
     if every > 1:
         lat.set_closure_interval(every)
     if shift:
         lat.set_closure_shift(shift)
     co = pyclosure.ClosureOracle(lattice, L, tau, c_smag, p, MU, ISD, out[0], out[1], band[0], band[1],
-                                 force=force, solid=solid, every=every, shift=shift)
+                                 force=force, solid=solid, every=every, shift=shift, fp16=fp16)
     co.init_equilibrium(rho, u)
     return lat, m, co
 
 
+@pytest.mark.parametrize("fp16", [False, True], ids=["bf16", "fp16"])
 @pytest.mark.parametrize("lattice", [lbmc.D3Q15, lbmc.D2Q9])
-def test_closure_single_call_matches_oracle(lattice):
+def test_closure_single_call_matches_oracle(lattice, fp16):
     """Call 0 (collision in place) from equilibrium plus a random second-order
     non-equilibrium part f_i^neq = 4.5 w_i (e_i e_i - I/3) : A (zero mass and
     momentum, Pi = A): post-collision populations at every node whose network
     tau is not within 1e-3 of a band edge (where a last-bit difference may
     pick the other branch), with both branches present."""
-    print(f"[closure] single call lattice={lattice}", flush=True)
+    print(f"[closure] single call lattice={lattice} fp16={fp16}", flush=True)
     if lattice == lbmc.D3Q15:
         L, tau, cs, force, solid = (16, 16, 16), 0.51, 0.17, None, None
     else:
@@ -187,7 +196,8 @@ def test_closure_single_call_matches_oracle(lattice):
     band = (tau + 0.005, tau + 0.04)
     # network output y has mean ~0.07 and spread ~0.16 here: tau' = tau + 0.02
     # + 0.1 y leaves the band on both sides at some nodes
-    lat, m, co = _pair(lattice, L, tau, cs, lbmc.FP32, band, (0.1, tau + 0.02), force=force, solid=solid)
+    lat, m, co = _pair(lattice, L, tau, cs, lbmc.FP32, band, (0.1, tau + 0.02), force=force, solid=solid,
+                       fp16=fp16)
     dim = co.dim
     n = int(np.prod(L))
     rng = np.random.default_rng(77)
@@ -211,13 +221,13 @@ def test_closure_single_call_matches_oracle(lattice):
     m.close()
 
 
-def _run_pair(lattice, L, tau, c_smag, storage, steps, force=None, solid=None, every=1, shift=0.0):
+def _run_pair(lattice, L, tau, c_smag, storage, steps, force=None, solid=None, every=1, shift=0.0, fp16=False):
     """Network output well inside the band everywhere (no branch near its
     edges): the macros after `steps` calls.  With a shift, the model's own
     output is centred on tau + 0.03 - shift."""
     band = (tau + 0.001, tau + 0.06)
     lat, m, co = _pair(lattice, L, tau, c_smag, storage, band, (0.004, tau + 0.03 - shift), force=force,
-                       solid=solid, every=every, shift=shift)
+                       solid=solid, every=every, shift=shift, fp16=fp16)
     lat.run(0, steps, 1)
     rd, ud = lat.get_macros()
     ro, uo = co.run(steps)
@@ -227,10 +237,12 @@ def _run_pair(lattice, L, tau, c_smag, storage, steps, force=None, solid=None, e
     return (rd, ud), (ro, uo), co.fallback.mean(), edge.min()
 
 
-@pytest.mark.parametrize("storage,tol", [(lbmc.FP32, 1e-5), (lbmc.FP16S, 2e-4), (lbmc.BF16S, 2e-3)])
-def test_closure_steps_d3q15_match_oracle(storage, tol):
-    print(f"[closure] 12 calls D3Q15 16^3 storage={storage}", flush=True)
-    (rd, ud), (ro, uo), fb, edge = _run_pair(lbmc.D3Q15, (16, 16, 16), 0.51, 0.17, storage, 12)
+@pytest.mark.parametrize("storage,tol,fp16", [(lbmc.FP32, 1e-5, False), (lbmc.FP16S, 2e-4, False),
+                                              (lbmc.BF16S, 2e-3, False), (lbmc.FP32, 1e-5, True)],
+                         ids=["fp32", "fp16s", "bf16s", "fp32-fp16net"])
+def test_closure_steps_d3q15_match_oracle(storage, tol, fp16):
+    print(f"[closure] 12 calls D3Q15 16^3 storage={storage} fp16={fp16}", flush=True)
+    (rd, ud), (ro, uo), fb, edge = _run_pair(lbmc.D3Q15, (16, 16, 16), 0.51, 0.17, storage, 12, fp16=fp16)
     assert fb == 0.0 and edge > 1e-3
     assert np.max(np.abs(ud - uo)) < tol
     assert np.max(np.abs(rd - ro)) < tol
