@@ -368,8 +368,8 @@ def _f32(a, shape_tail=()):
 
 class Mlp:
     """The closure network on one device: FP32 master weights, Adam state and
-    the operand images the kernels read (fp16 for the closure steps and
-    forward, bf16 for training).
+    the operand images the kernels read (bf16; an fp16 evaluation image for
+    set_eval_fp16).
 
     params: LBMC_MLP_NPARAM floats (w1[64][8] with the bias in column 7,
     w2[64][64], b2[64], w3[64], b3).  Inputs are normalised as
